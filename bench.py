@@ -1,0 +1,245 @@
+"""Benchmark: 3-D staggered-grid CPML time stepping on one MI355X.
+
+A "step" is one full reference time step (moment-tensor injection, velocity
+sweep, stress sweep, receiver gather) of the Elastic3D_SSG_Tensor class on a
+synthetic model that is resident in HBM before the timed region starts.
+
+Default workload (cfg2): 200^3 cells, 10 m spacing, homogeneous vp=3500,
+vs=2000, rho=2200, dt=1 ms, 20-cell CPML on six faces, seeded unit-norm moment
+tensor (10 Hz Ricker) at the centre, 50x50 receivers at k = s+5; 1500 steps.
+
+Prints one JSON line: Gcells/s (nx*ny*nz*steps/t), the roofline of the
+dominant kernel (algorithmic HBM bytes per launch / measured launch time
+against 8 TB/s) and a CPU baseline (the oracle's C restatement with OpenMP on
+this host's cores, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def pml_params(vp_max, vs_max, npml, d, f, kmax=None):
+    return {"vp_max": vp_max, "pml_x_thick": npml * d[0], "pml_y_thick": npml * d[1],
+            "pml_z_thick": npml * d[2], "Rcoef": 1e-6, "theta": 1, "alpha_max_pml": np.pi * f,
+            "kmax_pml": kmax if kmax is not None else vs_max / (5 * d[0] * f)}
+
+
+def unit_mt(seed):
+    m = np.random.default_rng(seed).standard_normal(6)
+    m /= np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2 + 2 * (m[3] ** 2 + m[4] ** 2 + m[5] ** 2))
+    return m.astype(np.float32)
+
+
+def workload(name, nt, seed=0):
+    """Model, geometry and PML of a named workload (host arrays)."""
+    if name in ("cfg2", "cfg2_spml"):
+        n = 200
+        shape = (n, n, n)
+        vs = np.full(shape, 2000.0, np.float32)
+        vp = np.full(shape, 3500.0, np.float32)
+        rho = np.full(shape, 2200.0, np.float32)
+        d, dt, f, npml = (10.0, 10.0, 10.0), 1e-3, 10.0, 20
+        src = (100, 100, 100)
+        lin = np.round(np.linspace(3, n - 4, 50)).astype(np.int64)
+        rx, ry = (a.ravel() for a in np.meshgrid(lin, lin, indexing="ij"))
+        pml = pml_params(3500.0, 2000.0, npml, d, f, kmax=4.0)
+    elif name == "cfg3":
+        from seiswave_taichi_amd.models import layered_folded_model
+        shape = (600, 600, 400)
+        d, f, npml = (12.5, 12.5, 12.5), 10.0, 20
+        vs, vp, rho = layered_folded_model(shape, d[0], seed=seed)
+        vp_max = float(vp.max())
+        dt = 0.45 * d[0] / (vp_max * np.sqrt(3.0))
+        src = (300, 300, 200)
+        lin = np.round(np.linspace(3, shape[0] - 4, 200)).astype(np.int64)
+        rx, ry = (a.ravel() for a in np.meshgrid(lin, lin, indexing="ij"))
+        pml = pml_params(vp_max, float(vs.max()), npml, d, f)
+    else:
+        raise ValueError(name)
+    rz = np.full_like(rx, 2 + 5)
+    return dict(shape=shape, vs=vs, vp=vp, rho=rho, d=d, dt=dt, f=f, src=src, rec=(rx, ry, rz), pml=pml,
+                mt=unit_mt(seed), nt=nt)
+
+
+def algorithmic_bytes(shape, s, slabs, spml=False):
+    """Compulsory HBM bytes of one step (two-sweep schedule, f32).
+
+    Per updated cell: velocity sweep 13 words (6 stresses, 3 velocities read,
+    3 written, rho), stress sweep 17 words (3 velocities, 6 stresses read, 6
+    written, lambda, mu).  Per (updated cell, PML direction) pair: 3 + 3 memory
+    variables read and written (48 B; split-field PML 64 B).  N_u counts
+    prod(n_d - 2s - 1) cells; S_u the updated cells whose d index is in the
+    d slab, summed over d.
+    """
+    core = [n - 2 * s - 1 for n in shape]
+    n_u = int(np.prod(core))
+    s_u = 0
+    for d, (lo, hi) in enumerate(slabs):
+        n = shape[d]
+        w = len([i for i in range(s, n - s - 1) if i < lo or i >= hi])
+        s_u += w * n_u // core[d]
+    per_pair = 64 if spml else 48
+    vel = 52 * n_u + per_pair // 2 * s_u
+    stress = 68 * n_u + per_pair // 2 * s_u
+    return n_u, s_u, vel, stress
+
+
+def traffic_from_profile(kernel_tag, workload_name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{workload_name}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
+def cpu_baseline(w, budget_s=15.0):
+    """Oracle (C restatement, OpenMP) on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    cores = min(aff, int(env)) if env and env.isdigit() else aff
+    O.set_threads(cores)
+    rx, ry, rz = w["rec"]
+    o = O.Oracle3D("tensor", w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], w["src"],
+                   np.stack([rx, ry, rz], 1), 64, 2, freq=w["f"], mt=w["mt"])
+    o.set_pml([True] * 6, w["pml"])
+    o.run(2)  # warm-up
+    t0 = time.perf_counter()
+    o.step(2)
+    t1 = time.perf_counter() - t0
+    nsteps = int(max(2, min(40, budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    o.run(nsteps, it0=3)
+    el = time.perf_counter() - t0
+    cells = int(np.prod(w["shape"]))
+    return {"value": round(cells * nsteps / el / 1e9, 4), "unit": "Gcells/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{nsteps} steps of the same workload after 3 warm-up steps, f32, "
+                      f"{cores} OpenMP threads ({el:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="cfg2", choices=["cfg2", "cfg2_spml", "cfg3"])
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU domain decomposition is "
+                                   "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd.Elastic3D_SSG_SPML import ElasticWAVE as SPML
+    from seiswave_taichi_amd.Elastic3D_SSG_Tensor import ElasticWAVE as Tensor
+    from seiswave_taichi_amd.coeffs import ricker2
+
+    torch.cuda.set_device(0)
+    nt = args.warmup + args.steps
+    w = workload(args.config, nt)
+    nx, ny, nz = w["shape"]
+    rx, ry, rz = w["rec"]
+    amps = ricker2(np.arange(nt), w["dt"], 1.5 / w["f"], w["f"], 1.0)
+    src = {"kind": "moment", "cell": w["src"], "mt": w["mt"], "wavelet": amps}
+    spml = args.config == "cfg2_spml"
+    if spml:
+        sim = SPML(w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], *w["src"], 0, 0, 0, nt, 2, w["f"],
+                   source=src, receivers=np.stack([rx, ry, rz], 1))
+    else:
+        mt3 = np.zeros((3, 3), np.float32)
+        m = w["mt"]
+        mt3[0, 0], mt3[1, 1], mt3[2, 2], mt3[0, 1], mt3[0, 2], mt3[1, 2] = m
+        sim = Tensor(w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], *w["src"], rx[None], ry[None], rz[None],
+                     mt3, 1.0, nt, 2, w["f"], source=src)
+    sim.SetADEPML3D([True] * 6, w["pml"])
+    torch.cuda.synchronize()
+
+    sim.run(args.warmup, 0)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    sim.run(args.steps, args.warmup)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    t = ms / 1e3
+    cells = nx * ny * nz
+    gcells = cells * args.steps / t / 1e9
+    trace = sim.data.to_numpy()
+    if not np.all(np.isfinite(trace)):
+        raise RuntimeError("non-finite seismogram")
+
+    # per-kernel launch times (same stream, HIP events), for the roofline
+    e = sim._e
+    g = e.descriptor()
+    stream = e.stream()
+    kt = {}
+    for name, fn in (("velocity", L.load().sw_ssg3d_velocity_spml if spml else L.load().sw_ssg3d_velocity),
+                     ("stress", L.load().sw_ssg3d_stress_spml if spml else L.load().sw_ssg3d_stress)):
+        for _ in range(5):
+            L.check(fn(g, stream), name)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.kernel_reps):
+            L.check(fn(g, stream), name)
+        b.record()
+        torch.cuda.synchronize()
+        kt[name] = a.elapsed_time(b) / args.kernel_reps / 1e3
+    slabs = [(0, n) if spml else (ax.lo, ax.hi) for ax, n in zip(e.pml, e.gridsize)]
+    if spml:
+        slabs = [(ax.lo, ax.hi) for ax in e.pml]
+    n_u, s_u, b_vel, b_str = algorithmic_bytes(w["shape"], 2, slabs, spml)
+    dom = "stress" if kt["stress"] * 1 >= kt["velocity"] else "velocity"
+    b_dom = b_str if dom == "stress" else b_vel
+    ach = b_dom / kt[dom] / 1e9
+    tag = {"stress": "ssg3d_stress", "velocity": "ssg3d_velocity"}[dom] + ("_spml" if spml else "")
+    traffic = traffic_from_profile(tag, args.config)
+    step_bytes = b_vel + b_str
+    out = {
+        "metric": "3D SSG Gcells/s",
+        "value": round(gcells, 3),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {nx}x{ny}x{nz} Elastic3D_SSG_"
+                               f"{'SPML' if spml else 'Tensor'} CPML20, order 4, moment tensor, "
+                               f"{len(rx)} receivers", "grid": [nx, ny, nz], "accuracy": 2,
+                   "pml_cells": 20, "receivers": int(len(rx))},
+        "roofline": {"bound": "hbm", "kernel": tag, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": b_dom, "launch_ms": round(kt[dom] * 1e3, 4)},
+        "step_hbm_frac": round(step_bytes / (ms / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels_ms": {k: round(v * 1e3, 4) for k, v in kt.items()},
+        "bytes_per_step": step_bytes, "N_u": n_u, "S_u": s_u,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

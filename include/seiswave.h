@@ -1,0 +1,180 @@
+/*
+ * seiswave.h -- C ABI of libseiswave.so, the MI355X (gfx950) elastic
+ * staggered-grid time stepper.
+ *
+ * The library replaces the per-step Taichi kernels of SeisWave_Taichi's
+ * ElasticWAVE classes (paths relative to the reference root):
+ *   src/Elastic3D_SSG_CPML.py    update_SSG        :144-309
+ *   src/Elastic3D_SSG_Tensor.py  update_SSG        :166-345
+ *   src/Elastic3D_SSG_SPML.py    update_SSG_SPML   :160-393
+ *   src/Elastic2D_CPML.py        update_SSG_VS     :127-200
+ *                                update_SSG_SV     :202-274
+ *                                update_RSG        :276-353
+ *                                AEA_init / AEA    :112-125
+ * The reference binds these through Taichi's @ti.kernel launch; here the
+ * Python classes in seiswave_taichi_amd bind them through ctypes.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Every pointer in a descriptor is a
+ *     device pointer owned by the caller; the library never frees caller
+ *     memory and allocates nothing on the step path.
+ *   - Device layout is x fastest: a 3-D field (nx,ny,nz) is stored as
+ *     [nz][ny][nx], element (i,j,k) at (k*ny + j)*nx + i; a 2-D field
+ *     (nx,nz) as [nz][nx], element (i,j) at j*nx + i.
+ *   - Work is enqueued on `stream` (a hipStream_t passed as void*; NULL is
+ *     the default stream).  No call synchronises the device.
+ *   - Every function returns 0 on success and a negative SW_E* code on
+ *     failure; sw_last_error() returns the message of the calling thread's
+ *     last failure.
+ */
+#ifndef SEISWAVE_H
+#define SEISWAVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW_MAX_ORDER 8
+
+#define SW_OK 0
+#define SW_EINVAL (-1) /* invalid argument or shape                      */
+#define SW_EHIP (-2)   /* HIP runtime error (launch or device)            */
+
+/* Which reference class's arithmetic a 3-D descriptor follows.  They differ
+ * in the term order of the vz sum (src/Elastic3D_SSG_CPML.py:212 sums z,x,y;
+ * src/Elastic3D_SSG_Tensor.py:243 sums x,y,z). */
+#define SW_CLASS_CPML 0
+#define SW_CLASS_TENSOR 1
+#define SW_CLASS_SPML 2
+
+/* One axis of PML coefficients as SetADEPML3D leaves them
+ * (src/Elastic3D_SSG_CPML.py:364-522), all device arrays of length n.
+ * a/b/k drive the CPML memory variables; d drives the split-field PML.
+ * Memory variables of this axis exist only for indices [0,lo) and [hi,n):
+ * outside that range the coefficients are the identity (a = 0, k = 1) and the
+ * reference's memory variable stays exactly 0. */
+typedef struct sw_axis {
+    const float *a, *b, *k;       /* integer points */
+    const float *a_h, *b_h, *k_h; /* half points    */
+    const float *d, *d_h;         /* damping profile (split-field PML) */
+    const uint8_t *in, *in_h;     /* 1 where the kernel's f32 coordinate test puts the point in this axis's PML */
+    int32_t lo, hi;
+} sw_axis;
+
+/* 3-D state.  mem_x holds 6 slab arrays [nz][ny][wx] back to back
+ * (wx = px.lo + nx - px.hi), in the order
+ *   x: sxx_dx sxy_dx sxz_dx dvx_dx dvy_dx dvz_dx
+ * mem_y holds 6 arrays [nz][wy][nx]:
+ *   y: sxy_dy syy_dy syz_dy dvy_dy dvx_dy dvz_dy
+ * mem_z holds 6 arrays [wz][ny][nx]:
+ *   z: sxz_dz syz_dz szz_dz dvz_dz dvx_dz dvy_dz
+ * split (SPML only) holds 24 full-grid arrays in the order
+ *   vx_x vx_y vx_z vy_x vy_y vy_z vz_x vz_y vz_z sxx_x sxx_y sxx_z
+ *   syy_x syy_y syy_z sxy_x sxy_y sxz_x sxz_z syz_y syz_z szz_x szz_y szz_z */
+typedef struct sw_ssg3d {
+    int32_t nx, ny, nz, order; /* order = accuracy N (half stencil), 1..SW_MAX_ORDER */
+    int32_t cls;               /* SW_CLASS_* */
+    float dx, dy, dz, dt;
+    float c[SW_MAX_ORDER]; /* staggered coefficients, f32 (src/Elastic3D_SSG_CPML.py:311-322) */
+    const float *rho, *lam, *mu;
+    float *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;
+    sw_axis px, py, pz;
+    float *mem_x, *mem_y, *mem_z;
+    float *split;
+} sw_ssg3d;
+
+/* Source kinds. */
+#define SW_SRC_NONE 0
+#define SW_SRC_FORCE_VZ 1  /* vz[p] += dt*amp[it]/rho[p] after the velocity sweep
+                              (src/Elastic3D_SSG_CPML.py:245, src/Elastic3D_SSG_SPML.py:294) */
+#define SW_SRC_MOMENT 2    /* s_ab[p] += amp[it]*(-M_ab) before the velocity sweep
+                              (src/Elastic3D_SSG_Tensor.py:186-193) */
+#define SW_SRC_EXPLOSIVE 3 /* sxx,syy,szz[p] += amp[it]*(-1/sqrt(3)) before the velocity sweep
+                              (MOMENT with M = I/sqrt(3); no reference counterpart) */
+#define SW_SRC_FORCE_VX 4  /* vx[p] += dt*amp[it]/rho[p] after the velocity sweep
+                              (2-D SV, src/Elastic2D_CPML.py:245) */
+
+typedef struct sw_source {
+    int32_t kind;
+    int32_t ix, iy, iz; /* cell (iy unused in 2-D) */
+    float mt[6];        /* Mxx Myy Mzz Mxy Mxz Myz */
+    const float *amp;   /* device, one amplitude per step index */
+    int32_t namp;
+} sw_source;
+
+/* Receiver gather: data[it*nrec + r] = field[idx[r]] after the stress sweep
+ * (src/Elastic3D_SSG_Tensor.py:344-345).  field: 0 vx 1 vy 2 vz 3 sxx 4 syy
+ * 5 szz 6 sxy 7 sxz 8 syz (2-D: 0 vx 2 vz 3 sxx 5 szz 7 sxz). */
+typedef struct sw_receivers {
+    int32_t nrec;
+    int32_t ncomp;          /* number of recorded components (1..9)       */
+    int32_t comp[9];        /* field code per component                   */
+    const int32_t *idx;     /* device, nrec linear device indices         */
+    float *data;            /* device, [nt][ncomp][nrec]                  */
+    int32_t nt;
+} sw_receivers;
+
+/* ---- library ---- */
+int sw_version(void);                        /* (major<<16)|(minor<<8)|patch */
+int sw_last_error(char *buf, size_t n);      /* copies the message; returns its length */
+int sw_device_init(int device);              /* selects and warms the device; idempotent */
+
+/* ---- 3-D sweeps (one reference loop group each) ---- */
+/* Velocity loops of update_SSG: src/Elastic3D_SSG_CPML.py:164-212,
+ * src/Elastic3D_SSG_Tensor.py:195-243.  Requires cls CPML or TENSOR. */
+int sw_ssg3d_velocity(const sw_ssg3d *g, void *stream);
+/* Stress loops of update_SSG: src/Elastic3D_SSG_CPML.py:247-307,
+ * src/Elastic3D_SSG_Tensor.py:246-310. */
+int sw_ssg3d_stress(const sw_ssg3d *g, void *stream);
+/* Split-field velocity / stress loops: src/Elastic3D_SSG_SPML.py:182-259, :296-391. */
+int sw_ssg3d_velocity_spml(const sw_ssg3d *g, void *stream);
+int sw_ssg3d_stress_spml(const sw_ssg3d *g, void *stream);
+/* Source injection for step it (the serial statements of the reference kernels). */
+int sw_ssg3d_inject(const sw_ssg3d *g, const sw_source *src, int32_t it, void *stream);
+/* Receiver gather for step it (src/Elastic3D_SSG_CPML.py:309, Tensor :344-345). */
+int sw_ssg3d_record(const sw_ssg3d *g, const sw_receivers *rec, int32_t it, void *stream);
+/* nsteps full reference steps it0..it0+nsteps-1 in the order of g->cls:
+ *   CPML/SPML: velocity, force, stress, record;  TENSOR: moment, velocity, stress, record.
+ * rec may be NULL. */
+int sw_ssg3d_run(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec,
+                 int32_t it0, int32_t nsteps, void *stream);
+
+/* ---- 2-D (src/Elastic2D_CPML.py) ---- */
+#define SW_2D_VS 0  /* update_SSG_VS :127-200 */
+#define SW_2D_SV 1  /* update_SSG_SV :202-274 */
+#define SW_2D_RSG 2 /* update_RSG    :276-353 */
+
+/* 2-D state.  mem holds 10 full-grid arrays [nz][nx] back to back:
+ *   sxx_dx sxx_dz sxz_dx sxz_dz szz_dx szz_dz dvx_dx dvx_dz dvz_dx dvz_dz */
+typedef struct sw_ssg2d {
+    int32_t nx, nz, order, scheme; /* scheme = SW_2D_* */
+    float dx, dz, dt;
+    float c[SW_MAX_ORDER];
+    float *rho, *lam, *mu;
+    float *vx, *vz, *sxx, *szz, *sxz;
+    sw_axis px, pz; /* py unused in 2-D */
+    float *mem;
+} sw_ssg2d;
+
+int sw_ssg2d_velocity(const sw_ssg2d *g, void *stream);
+int sw_ssg2d_stress(const sw_ssg2d *g, void *stream);
+int sw_ssg2d_inject(const sw_ssg2d *g, const sw_source *src, int32_t it, void *stream);
+int sw_ssg2d_record(const sw_ssg2d *g, const sw_receivers *rec, int32_t it, void *stream);
+/* nsteps reference steps: velocity, source, stress, record.  For SW_2D_RSG the
+ * source is the 2x2 spread of src/Elastic2D_CPML.py:322-323 with amp[it]. */
+int sw_ssg2d_run(const sw_ssg2d *g, const sw_source *src, const sw_receivers *rec,
+                 int32_t it0, int32_t nsteps, void *stream);
+/* Free surface (src/Elastic2D_CPML.py:113-125): lam = 0, rho *= 0.5 on planes
+ * i < order; and per step sxx = 0 on plane i = order. */
+int sw_ssg2d_free_surface_init(const sw_ssg2d *g, void *stream);
+int sw_ssg2d_free_surface(const sw_ssg2d *g, void *stream);
+/* 3-D variant (src/Elastic3D_SSG_Tensor.py:151-157): lam = 0, rho *= 0.5 on k < order. */
+int sw_ssg3d_free_surface_init(const sw_ssg3d *g, float *rho, float *lam, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""Shared 3-D engine behind the drop-in ElasticWAVE classes.
+
+Holds the device state (torch-ROCm tensors used only as device memory),
+builds the ``sw_ssg3d`` descriptor of include/seiswave.h and calls
+libseiswave.so through ctypes on torch's current stream.
+
+Reference anchors: constructor state src/Elastic3D_SSG_CPML.py:9-122,
+moduli :124-141, SetADEPML3D :324-522, update_SSG :144-309.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .coeffs import diff_coff
+from .field import Field, as_numpy, to_device_layout
+from .pml import axis_pml, check_parameter
+
+WAVEFIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def _check_fieldtype(fieldtype):
+    name = getattr(fieldtype, "__name__", None) or str(fieldtype)
+    if fieldtype in (None, np.float32, torch.float32) or "f32" in name or "float32" in name:
+        return
+    raise NotImplementedError(
+        f"fieldtype {fieldtype!r}: the MI355X kernels compute in f32 only (f64 exists in the CPU oracle)")
+
+
+def gpu_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("seiswave_taichi_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def moduli(vs: np.ndarray, vp: np.ndarray, rho: np.ndarray):
+    """mu = rho*vs**2, lam = rho*(vp**2 - 2*vs**2) in NumPy f32, as the
+    reference computes them (src/Elastic3D_SSG_CPML.py:124-141)."""
+    vs, vp, rho = (np.asarray(a, dtype=np.float32) for a in (vs, vp, rho))
+    mu = rho * vs ** 2
+    lam = rho * (vp ** 2 - 2 * vs ** 2)
+    return mu.astype(np.float32), lam.astype(np.float32)
+
+
+class SSG3DEngine:
+    """Device state and launches for one 3-D ElasticWAVE instance."""
+
+    def __init__(self, vs, vp, rho, dx, dy, dz, dt, accuracy, nt, cls):
+        _check_dims = np.shape(vs)
+        if len(_check_dims) != 3:
+            raise ValueError(f"3-D solver needs (nx, ny, nz) model arrays, got shape {_check_dims}")
+        self.lib = L.load()
+        self.device = gpu_device()
+        self.cls = cls
+        self.gridsize = tuple(int(v) for v in _check_dims)
+        self.nx, self.ny, self.nz = self.gridsize
+        self.star = int(accuracy)
+        if not 1 <= self.star <= L.SW_MAX_ORDER:
+            raise ValueError(f"accuracy must be in 1..{L.SW_MAX_ORDER}, got {accuracy}")
+        need = 2 * self.star + 2
+        if min(self.gridsize) < need:
+            raise ValueError(f"grid {self.gridsize} too small for accuracy {accuracy} (each axis >= {need})")
+        self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
+        self.nt = int(nt)
+        self.c = diff_coff(self.star)
+        vs_np, vp_np, rho_np = (as_numpy(a) for a in (vs, vp, rho))
+        for name, a in (("vs", vs_np), ("vp", vp_np), ("rho", rho_np)):
+            if a.shape != self.gridsize:
+                raise ValueError(f"{name} has shape {a.shape}, expected {self.gridsize}")
+        mu_np, lam_np = moduli(vs_np, vp_np, rho_np)
+        dev = self.device
+        self.rho_t = to_device_layout(rho_np.astype(np.float32), dev)
+        self.mu_t = to_device_layout(mu_np, dev)
+        self.lam_t = to_device_layout(lam_np, dev)
+        shape = (self.nz, self.ny, self.nx)
+        self.f = {n: torch.zeros(shape, dtype=torch.float32, device=dev) for n in WAVEFIELDS}
+        self.pml = None
+        self._keep = []
+
+    # ------------------------------------------------------------- PML
+    def set_pml(self, pml_surface, parameter: dict):
+        """SetADEPML3D (src/Elastic3D_SSG_CPML.py:324-522)."""
+        check_parameter(parameter, ("pml_x_thick", "pml_y_thick", "pml_z_thick"))
+        if len(pml_surface) != 6:
+            raise ValueError("pml_surface needs 6 flags: x-left, x-right, y-left, y-right, z-up, z-bottom")
+        p = parameter
+        axes = []
+        for n, d, thick, lo, hi in zip(self.gridsize, (self.dx, self.dy, self.dz),
+                                       (p["pml_x_thick"], p["pml_y_thick"], p["pml_z_thick"]),
+                                       pml_surface[0::2], pml_surface[1::2]):
+            axes.append(axis_pml(n, self.star, d, self.dt, thick, p["vp_max"], p["Rcoef"],
+                                 p["alpha_max_pml"], p["kmax_pml"], p["theta"], bool(lo), bool(hi)))
+        self.pml = axes
+        dev = self.device
+        self.pml_dev = []
+        for ax in axes:
+            coeff = torch.from_numpy(np.stack([ax.a, ax.b, ax.k, ax.a_h, ax.b_h, ax.k_h, ax.d, ax.d_h])).to(dev)
+            flags = torch.from_numpy(np.stack([ax.inside, ax.inside_h])).to(dev)
+            self.pml_dev.append((coeff.contiguous(), flags.contiguous()))
+        self._alloc_boundary_state()
+
+    def _alloc_boundary_state(self):
+        dev = self.device
+        nx, ny, nz = self.nx, self.ny, self.nz
+        ax, ay, az = self.pml
+        if self.cls == L.SW_CLASS_SPML:
+            self.split = torch.zeros((24, nz, ny, nx), dtype=torch.float32, device=dev)
+            self.mem = None
+        else:
+            self.split = None
+            self.mem = (
+                torch.zeros((6, nz, ny, ax.width), dtype=torch.float32, device=dev),
+                torch.zeros((6, nz, ay.width, nx), dtype=torch.float32, device=dev),
+                torch.zeros((6, az.width, ny, nx), dtype=torch.float32, device=dev),
+            )
+
+    def reset(self):
+        """Zero the wavefields and boundary state (a fresh run, same model)."""
+        for t in self.f.values():
+            t.zero_()
+        if self.split is not None:
+            self.split.zero_()
+        if self.mem is not None:
+            for t in self.mem:
+                t.zero_()
+
+    # ------------------------------------------------------- descriptor
+    def descriptor(self) -> L.SwSSG3D:
+        if self.pml is None:
+            raise RuntimeError("SetADEPML3D must be called before the first time step "
+                               "(the reference raises AttributeError on ymin_pml otherwise)")
+        g = L.SwSSG3D()
+        g.nx, g.ny, g.nz, g.order, g.cls = self.nx, self.ny, self.nz, self.star, self.cls
+        g.dx, g.dy, g.dz, g.dt = float(self.dx), float(self.dy), float(self.dz), float(self.dt)
+        for i, v in enumerate(self.c):
+            g.c[i] = float(v)
+        g.rho, g.lam, g.mu = (L.fptr(t) for t in (self.rho_t, self.lam_t, self.mu_t))
+        for n in WAVEFIELDS:
+            setattr(g, n, L.fptr(self.f[n]))
+        for name, ax, (coeff, flags) in zip(("px", "py", "pz"), self.pml, self.pml_dev):
+            a = L.SwAxis()
+            base = coeff.data_ptr()
+            n = coeff.shape[1]
+            for r, fname in enumerate(("a", "b", "k", "a_h", "b_h", "k_h", "d", "d_h")):
+                setattr(a, fname, C.cast(C.c_void_p(base + 4 * r * n), L._fp))
+            a.in_ = C.cast(C.c_void_p(flags.data_ptr()), L._u8p)
+            a.in_h = C.cast(C.c_void_p(flags.data_ptr() + n), L._u8p)
+            if self.cls == L.SW_CLASS_SPML:
+                a.lo, a.hi = 0, n
+            else:
+                a.lo, a.hi = ax.lo, ax.hi
+            setattr(g, name, a)
+        if self.mem is not None:
+            g.mem_x, g.mem_y, g.mem_z = (L.fptr(t) for t in self.mem)
+        if self.split is not None:
+            g.split = L.fptr(self.split)
+        return g
+
+    def stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def device_index(self, i, j, k) -> int:
+        return (int(k) * self.ny + int(j)) * self.nx + int(i)
+
+    def field(self, name) -> Field:
+        return Field(self.f[name])
+
+
+class Source:
+    """Device-side source description (``sw_source``)."""
+
+    def __init__(self, engine, kind, cell, amps: np.ndarray, mt=None):
+        self.kind = kind
+        self.cell = tuple(int(v) for v in cell)
+        self.amp_t = torch.from_numpy(np.ascontiguousarray(amps, dtype=np.float32)).to(engine.device)
+        self.mt = np.zeros(6, dtype=np.float32) if mt is None else np.asarray(mt, dtype=np.float32)
+        self.s = L.SwSource()
+        self.s.kind = kind
+        self.s.ix, self.s.iy, self.s.iz = self.cell
+        for i, v in enumerate(self.mt):
+            self.s.mt[i] = float(v)
+        self.s.amp = L.fptr(self.amp_t)
+        self.s.namp = len(amps)
+        shape = engine.gridsize
+        if kind != L.SW_SRC_NONE and not all(0 <= c < n for c, n in zip(self.cell, shape)):
+            raise IndexError(f"source cell {self.cell} outside grid {shape}")
+
+
+class Receivers:
+    """Device-side receiver gather (``sw_receivers``): data is [nt][ncomp][nrec]."""
+
+    def __init__(self, engine, cells: np.ndarray, nt: int, components=("vx",)):
+        cells = np.asarray(cells, dtype=np.int64).reshape(-1, 3)
+        shape = engine.gridsize
+        for d in range(3):
+            if len(cells) and (cells[:, d].min() < 0 or cells[:, d].max() >= shape[d]):
+                raise IndexError(f"receiver cells outside grid {shape}")
+        idx = (cells[:, 2] * engine.ny + cells[:, 1]) * engine.nx + cells[:, 0]
+        self.cells = cells
+        self.components = tuple(components)
+        self.idx_t = torch.from_numpy(idx.astype(np.int32)).to(engine.device)
+        self.data_t = torch.zeros((nt, len(self.components), len(cells)), dtype=torch.float32,
+                                  device=engine.device)
+        r = L.SwReceivers()
+        r.nrec, r.ncomp, r.nt = len(cells), len(self.components), nt
+        for i, cname in enumerate(self.components):
+            r.comp[i] = L.FIELD_CODE[cname]
+        r.idx = L.i32ptr(self.idx_t)
+        r.data = L.fptr(self.data_t)
+        self.r = r

@@ -1,0 +1,419 @@
+// C ABI of libseiswave.so: argument checks, source injection, receiver
+// gather, multi-step drivers and the free-surface helpers.
+#include "sw_internal.h"
+
+#include <cstring>
+
+namespace sw {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st);
+int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st);
+int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st);
+int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st);
+int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st);
+int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st);
+
+static int check_axis(const sw_axis &a, int n, const char *name, bool need_cpml, bool need_split)
+{
+    SW_REQUIRE(a.lo >= 0 && a.lo <= a.hi && a.hi <= n, "axis %s: slab bounds lo=%d hi=%d outside [0,%d]",
+               name, a.lo, a.hi, n);
+    if (need_cpml)
+        SW_REQUIRE(a.a && a.b && a.k && a.a_h && a.b_h && a.k_h, "axis %s: CPML coefficient arrays missing", name);
+    if (need_split)
+        SW_REQUIRE(a.d && a.d_h && a.in && a.in_h, "axis %s: split-field PML arrays missing", name);
+    return SW_OK;
+}
+
+int check_ssg3d(const sw_ssg3d *g, bool spml)
+{
+    SW_REQUIRE(g != nullptr, "null descriptor");
+    SW_REQUIRE(g->order >= 1 && g->order <= SW_MAX_ORDER, "order %d outside 1..%d", g->order, SW_MAX_ORDER);
+    const int need = 2 * g->order + 2;
+    SW_REQUIRE(g->nx >= need && g->ny >= need && g->nz >= need,
+               "grid %dx%dx%d too small for order %d (each axis needs >= %d)", g->nx, g->ny, g->nz,
+               g->order, need);
+    SW_REQUIRE((long)g->nx * g->ny * g->nz < (1L << 40), "grid too large");
+    SW_REQUIRE(g->rho && g->lam && g->mu, "model arrays missing");
+    SW_REQUIRE(g->vx && g->vy && g->vz && g->sxx && g->syy && g->szz && g->sxy && g->sxz && g->syz,
+               "wavefield arrays missing");
+    int rc;
+    if ((rc = check_axis(g->px, g->nx, "x", !spml, spml)) != SW_OK) return rc;
+    if ((rc = check_axis(g->py, g->ny, "y", !spml, spml)) != SW_OK) return rc;
+    if ((rc = check_axis(g->pz, g->nz, "z", !spml, spml)) != SW_OK) return rc;
+    if (spml) {
+        SW_REQUIRE(g->split != nullptr, "split-field arrays missing");
+    } else {
+        SW_REQUIRE(g->px.lo + g->nx - g->px.hi == 0 || g->mem_x, "x memory slab missing");
+        SW_REQUIRE(g->py.lo + g->ny - g->py.hi == 0 || g->mem_y, "y memory slab missing");
+        SW_REQUIRE(g->pz.lo + g->nz - g->pz.hi == 0 || g->mem_z, "z memory slab missing");
+    }
+    return SW_OK;
+}
+
+int check_ssg2d(const sw_ssg2d *g)
+{
+    SW_REQUIRE(g != nullptr, "null descriptor");
+    SW_REQUIRE(g->order >= 1 && g->order <= SW_MAX_ORDER, "order %d outside 1..%d", g->order, SW_MAX_ORDER);
+    SW_REQUIRE(g->scheme == SW_2D_VS || g->scheme == SW_2D_SV || g->scheme == SW_2D_RSG, "unknown 2-D scheme %d",
+               g->scheme);
+    const int need = 2 * g->order + 2;
+    SW_REQUIRE(g->nx >= need && g->nz >= need, "grid %dx%d too small for order %d", g->nx, g->nz, g->order);
+    SW_REQUIRE(g->rho && g->lam && g->mu && g->vx && g->vz && g->sxx && g->szz && g->sxz && g->mem,
+               "2-D arrays missing");
+    int rc;
+    if ((rc = check_axis(g->px, g->nx, "x", true, false)) != SW_OK) return rc;
+    if ((rc = check_axis(g->pz, g->nz, "z", true, false)) != SW_OK) return rc;
+    SW_REQUIRE(g->px.in && g->px.in_h && g->pz.in && g->pz.in_h, "2-D in-PML flags missing");
+    return SW_OK;
+}
+
+namespace {
+
+// ------------------------------------------------------------------ sources
+__global__ void k_inject3d(sw_ssg3d g, sw_source s, int it)
+{
+    const float amp = s.amp[it];
+    const long p = ((long)s.iz * g.ny + s.iy) * g.nx + s.ix;
+    switch (s.kind) {
+    case SW_SRC_FORCE_VZ: g.vz[p] += g.dt * amp / g.rho[p]; break;
+    case SW_SRC_FORCE_VX: g.vx[p] += g.dt * amp / g.rho[p]; break;
+    case SW_SRC_MOMENT:
+        g.sxx[p] += amp * (-s.mt[0]);
+        g.syy[p] += amp * (-s.mt[1]);
+        g.szz[p] += amp * (-s.mt[2]);
+        g.sxy[p] += amp * (-s.mt[3]);
+        g.sxz[p] += amp * (-s.mt[4]);
+        g.syz[p] += amp * (-s.mt[5]);
+        break;
+    case SW_SRC_EXPLOSIVE: {
+        const float e = amp * (-0.57735026918962576f);
+        g.sxx[p] += e;
+        g.syy[p] += e;
+        g.szz[p] += e;
+    } break;
+    default: break;
+    }
+}
+
+__global__ void k_inject2d(sw_ssg2d g, sw_source s, int it)
+{
+    const float amp = s.amp[it];
+    const long p = (long)s.iz * g.nx + s.ix;
+    if (g.scheme == SW_2D_RSG) {
+        // src/Elastic2D_CPML.py:322-323: vz[isx+a, isz+b] += dt*source/(8*rho), a,b in {-1,0}
+        for (int a = -1; a < 1; ++a)
+            for (int b = -1; b < 1; ++b) {
+                const long q = p + a + (long)b * g.nx;
+                g.vz[q] += g.dt * amp / (8.f * g.rho[q]);
+            }
+        return;
+    }
+    switch (s.kind) {
+    case SW_SRC_FORCE_VZ: g.vz[p] += g.dt * amp / g.rho[p]; break;
+    case SW_SRC_FORCE_VX: g.vx[p] += g.dt * amp / g.rho[p]; break;
+    case SW_SRC_MOMENT:  // 2-D moment tensor (Mxx, Mzz, Mxz taken from mt[0], mt[2], mt[4])
+        g.sxx[p] += amp * (-s.mt[0]);
+        g.szz[p] += amp * (-s.mt[2]);
+        g.sxz[p] += amp * (-s.mt[4]);
+        break;
+    case SW_SRC_EXPLOSIVE: {
+        const float e = amp * (-0.70710678118654752f);
+        g.sxx[p] += e;
+        g.szz[p] += e;
+    } break;
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------- receivers
+struct FieldPtrs {
+    const float *f[9];
+};
+
+__global__ void k_record(FieldPtrs fp, sw_receivers r, int it)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= r.nrec * r.ncomp) return;
+    const int comp = t / r.nrec, rr = t - comp * r.nrec;
+    r.data[((long)it * r.ncomp + comp) * r.nrec + rr] = fp.f[r.comp[comp]][r.idx[rr]];
+}
+
+int check_source3d(const sw_ssg3d *g, const sw_source *s, int it)
+{
+    SW_REQUIRE(s != nullptr, "null source");
+    if (s->kind == SW_SRC_NONE) return SW_OK;
+    SW_REQUIRE(s->kind == SW_SRC_FORCE_VZ || s->kind == SW_SRC_FORCE_VX || s->kind == SW_SRC_MOMENT ||
+                   s->kind == SW_SRC_EXPLOSIVE,
+               "unknown source kind %d", s->kind);
+    SW_REQUIRE(s->amp != nullptr, "source amplitude array missing");
+    SW_REQUIRE(it >= 0 && it < s->namp, "step %d outside the source amplitude array (%d steps)", it, s->namp);
+    SW_REQUIRE(s->ix >= 0 && s->ix < g->nx && s->iy >= 0 && s->iy < g->ny && s->iz >= 0 && s->iz < g->nz,
+               "source cell (%d,%d,%d) outside the %dx%dx%d grid", s->ix, s->iy, s->iz, g->nx, g->ny, g->nz);
+    return SW_OK;
+}
+
+int check_source2d(const sw_ssg2d *g, const sw_source *s, int it)
+{
+    SW_REQUIRE(s != nullptr, "null source");
+    if (s->kind == SW_SRC_NONE) return SW_OK;
+    SW_REQUIRE(s->amp != nullptr, "source amplitude array missing");
+    SW_REQUIRE(it >= 0 && it < s->namp, "step %d outside the source amplitude array (%d steps)", it, s->namp);
+    const int lo = g->scheme == SW_2D_RSG ? 1 : 0;
+    SW_REQUIRE(s->ix >= lo && s->ix < g->nx && s->iz >= lo && s->iz < g->nz,
+               "source cell (%d,%d) outside the %dx%d grid", s->ix, s->iz, g->nx, g->nz);
+    return SW_OK;
+}
+
+int check_receivers(const sw_receivers *r, int it)
+{
+    SW_REQUIRE(r != nullptr, "null receivers");
+    if (r->nrec == 0) return SW_OK;
+    SW_REQUIRE(r->nrec > 0 && r->idx && r->data, "receiver arrays missing");
+    SW_REQUIRE(r->ncomp >= 1 && r->ncomp <= 9, "ncomp %d outside 1..9", r->ncomp);
+    for (int c = 0; c < r->ncomp; ++c) SW_REQUIRE(r->comp[c] >= 0 && r->comp[c] < 9, "bad component code");
+    SW_REQUIRE(it >= 0 && it < r->nt, "step %d outside the receiver record (%d steps)", it, r->nt);
+    return SW_OK;
+}
+
+int record_launch(FieldPtrs fp, const sw_receivers *r, int it, hipStream_t st)
+{
+    if (r->nrec == 0) return SW_OK;
+    const int n = r->nrec * r->ncomp;
+    hipLaunchKernelGGL(k_record, dim3((n + 255) / 256), dim3(256), 0, st, fp, *r, it);
+    return launch_status("receiver gather");
+}
+
+FieldPtrs fields3d(const sw_ssg3d *g)
+{
+    return FieldPtrs{{g->vx, g->vy, g->vz, g->sxx, g->syy, g->szz, g->sxy, g->sxz, g->syz}};
+}
+
+FieldPtrs fields2d(const sw_ssg2d *g)
+{
+    return FieldPtrs{{g->vx, nullptr, g->vz, g->sxx, nullptr, g->szz, nullptr, g->sxz, nullptr}};
+}
+
+__global__ void k_free_surface_init3d(int nx, int ny, int order, float *rho, float *lam)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)nx * ny * order;
+    if (t >= n) return;  // planes k < order are the first order*ny*nx elements
+    lam[t] = 0.f;
+    rho[t] = 0.5f * rho[t];
+}
+
+__global__ void k_free_surface_init2d(int nx, int nz, int order, float *rho, float *lam)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nx * nz) return;
+    const int i = (int)(t % nx);
+    if (i >= order) return;
+    lam[t] = 0.f;
+    rho[t] = 0.5f * rho[t];
+}
+
+__global__ void k_free_surface2d(int nx, int nz, int order, float *sxx)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nz) sxx[(long)j * nx + order] = 0.f;
+}
+
+}  // namespace
+}  // namespace sw
+
+using namespace sw;
+
+extern "C" {
+
+int sw_version(void) { return (0 << 16) | (1 << 8) | 0; }
+
+int sw_last_error(char *buf, size_t n)
+{
+    const size_t len = strlen(g_err);
+    if (buf && n) {
+        strncpy(buf, g_err, n - 1);
+        buf[n - 1] = '\0';
+    }
+    return (int)len;
+}
+
+int sw_device_init(int device)
+{
+    SW_TRY_HIP(hipSetDevice(device));
+    SW_TRY_HIP(hipFree(nullptr));
+    return SW_OK;
+}
+
+int sw_ssg3d_velocity(const sw_ssg3d *g, void *stream)
+{
+    int rc = check_ssg3d(g, false);
+    if (rc) return rc;
+    SW_REQUIRE(g->cls == SW_CLASS_CPML || g->cls == SW_CLASS_TENSOR, "velocity sweep needs a CPML/Tensor descriptor");
+    return launch_ssg3d_velocity(g, as_stream(stream));
+}
+
+int sw_ssg3d_stress(const sw_ssg3d *g, void *stream)
+{
+    int rc = check_ssg3d(g, false);
+    if (rc) return rc;
+    return launch_ssg3d_stress(g, as_stream(stream));
+}
+
+int sw_ssg3d_velocity_spml(const sw_ssg3d *g, void *stream)
+{
+    int rc = check_ssg3d(g, true);
+    if (rc) return rc;
+    return launch_ssg3d_velocity_spml(g, as_stream(stream));
+}
+
+int sw_ssg3d_stress_spml(const sw_ssg3d *g, void *stream)
+{
+    int rc = check_ssg3d(g, true);
+    if (rc) return rc;
+    return launch_ssg3d_stress_spml(g, as_stream(stream));
+}
+
+int sw_ssg3d_inject(const sw_ssg3d *g, const sw_source *src, int32_t it, void *stream)
+{
+    int rc = check_source3d(g, src, it);
+    if (rc) return rc;
+    if (src->kind == SW_SRC_NONE) return SW_OK;
+    hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, as_stream(stream), *g, *src, it);
+    return launch_status("source injection");
+}
+
+int sw_ssg3d_record(const sw_ssg3d *g, const sw_receivers *rec, int32_t it, void *stream)
+{
+    int rc = check_receivers(rec, it);
+    if (rc) return rc;
+    return record_launch(fields3d(g), rec, it, as_stream(stream));
+}
+
+int sw_ssg3d_run(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, int32_t it0, int32_t nsteps,
+                 void *stream)
+{
+    const bool spml = g && g->cls == SW_CLASS_SPML;
+    int rc = check_ssg3d(g, spml);
+    if (rc) return rc;
+    SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");
+    if (nsteps == 0) return SW_OK;
+    if ((rc = check_source3d(g, src, it0)) || (rc = check_source3d(g, src, it0 + nsteps - 1))) return rc;
+    if (rec && ((rc = check_receivers(rec, it0)) || (rc = check_receivers(rec, it0 + nsteps - 1)))) return rc;
+    hipStream_t st = as_stream(stream);
+    const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
+    const FieldPtrs fp = fields3d(g);
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        if (pre && src->kind != SW_SRC_NONE) {
+            hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        }
+        if ((rc = spml ? launch_ssg3d_velocity_spml(g, st) : launch_ssg3d_velocity(g, st))) return rc;
+        if (!pre && src->kind != SW_SRC_NONE) {
+            hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        }
+        if ((rc = spml ? launch_ssg3d_stress_spml(g, st) : launch_ssg3d_stress(g, st))) return rc;
+        if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
+    }
+    return launch_status("ssg3d run");
+}
+
+int sw_ssg2d_velocity(const sw_ssg2d *g, void *stream)
+{
+    int rc = check_ssg2d(g);
+    if (rc) return rc;
+    return launch_ssg2d_velocity(g, as_stream(stream));
+}
+
+int sw_ssg2d_stress(const sw_ssg2d *g, void *stream)
+{
+    int rc = check_ssg2d(g);
+    if (rc) return rc;
+    return launch_ssg2d_stress(g, as_stream(stream));
+}
+
+int sw_ssg2d_inject(const sw_ssg2d *g, const sw_source *src, int32_t it, void *stream)
+{
+    int rc = check_ssg2d(g);
+    if (rc) return rc;
+    if ((rc = check_source2d(g, src, it))) return rc;
+    if (src->kind == SW_SRC_NONE) return SW_OK;
+    hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, as_stream(stream), *g, *src, it);
+    return launch_status("2-D source injection");
+}
+
+int sw_ssg2d_record(const sw_ssg2d *g, const sw_receivers *rec, int32_t it, void *stream)
+{
+    int rc = check_receivers(rec, it);
+    if (rc) return rc;
+    for (int c = 0; c < rec->ncomp; ++c)
+        SW_REQUIRE(fields2d(g).f[rec->comp[c]] != nullptr, "component %d does not exist in 2-D", rec->comp[c]);
+    return record_launch(fields2d(g), rec, it, as_stream(stream));
+}
+
+int sw_ssg2d_run(const sw_ssg2d *g, const sw_source *src, const sw_receivers *rec, int32_t it0, int32_t nsteps,
+                 void *stream)
+{
+    int rc = check_ssg2d(g);
+    if (rc) return rc;
+    SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");
+    if (nsteps == 0) return SW_OK;
+    if ((rc = check_source2d(g, src, it0)) || (rc = check_source2d(g, src, it0 + nsteps - 1))) return rc;
+    if (rec) {
+        if ((rc = check_receivers(rec, it0)) || (rc = check_receivers(rec, it0 + nsteps - 1))) return rc;
+        for (int c = 0; c < rec->ncomp; ++c)
+            SW_REQUIRE(fields2d(g).f[rec->comp[c]] != nullptr, "component %d does not exist in 2-D", rec->comp[c]);
+    }
+    hipStream_t st = as_stream(stream);
+    const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
+    const FieldPtrs fp = fields2d(g);
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        if (pre) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg2d_velocity(g, st))) return rc;
+        if (!pre && src->kind != SW_SRC_NONE) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg2d_stress(g, st))) return rc;
+        if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
+    }
+    return launch_status("ssg2d run");
+}
+
+int sw_ssg2d_free_surface_init(const sw_ssg2d *g, void *stream)
+{
+    SW_REQUIRE(g && g->rho && g->lam, "free-surface init needs rho and lam");
+    SW_REQUIRE(g->nx > 0 && g->nz > 0 && g->order >= 1 && g->order <= g->nx, "bad 2-D shape/order");
+    const long n = (long)g->nx * g->nz;
+    hipLaunchKernelGGL(k_free_surface_init2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream),
+                       g->nx, g->nz, g->order, g->rho, g->lam);
+    return launch_status("2-D free-surface init");
+}
+
+int sw_ssg2d_free_surface(const sw_ssg2d *g, void *stream)
+{
+    SW_REQUIRE(g && g->sxx, "free surface needs sxx");
+    SW_REQUIRE(g->nx > 0 && g->nz > 0 && g->order >= 0 && g->order < g->nx, "bad 2-D shape/order");
+    hipLaunchKernelGGL(k_free_surface2d, dim3((g->nz + 255) / 256), dim3(256), 0, as_stream(stream), g->nx, g->nz,
+                       g->order, g->sxx);
+    return launch_status("2-D free surface");
+}
+
+int sw_ssg3d_free_surface_init(const sw_ssg3d *g, float *rho, float *lam, void *stream)
+{
+    SW_REQUIRE(g && rho && lam, "null argument");
+    SW_REQUIRE(g->order >= 1 && g->order <= g->nz, "bad order");
+    const long n = (long)g->nx * g->ny * g->order;
+    hipLaunchKernelGGL(k_free_surface_init3d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream),
+                       g->nx, g->ny, g->order, rho, lam);
+    return launch_status("3-D free-surface init");
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+"""Host-side checks that need no GPU: the C ABI library, the ctypes mirror of
+the header, coefficient/PML host code and the Field wrapper."""
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from seiswave_taichi_amd import _lib as L
+from seiswave_taichi_amd.coeffs import diff_coff, diff_coff_op
+from seiswave_taichi_amd.field import Field, to_device_layout
+from seiswave_taichi_amd.pml import axis_pml, check_parameter, memory_slab
+from tests.conftest import ROOT, gpu_available
+
+HEADER = os.path.join(ROOT, "include", "seiswave.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^int\s+(sw_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = L.load()
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(L.SIGNATURES), "ctypes signature table out of sync with the header"
+
+
+def test_version_and_error_channel():
+    lib = L.load()
+    assert lib.sw_version() == (0 << 16) | (1 << 8)
+    # a null descriptor is rejected on the host, before any device work
+    rc = lib.sw_ssg3d_velocity(None, None)
+    assert rc == -1
+    assert "null descriptor" in L.last_error()
+
+
+def test_descriptor_checks_reject_bad_shapes():
+    lib = L.load()
+    g = L.SwSSG3D()
+    g.nx = g.ny = g.nz = 5
+    g.order = 2
+    assert lib.sw_ssg3d_stress(g, None) == -1
+    assert "too small" in L.last_error()
+    g.nx = g.ny = g.nz = 32
+    g.order = 9
+    assert lib.sw_ssg3d_stress(g, None) == -1
+    assert "order" in L.last_error()
+
+
+def test_ctypes_struct_sizes_match_header_layout():
+    # sw_axis: 8 float* + 2 uint8* + 2 int32 = 10*8 + 8 bytes
+    assert C_sizeof(L.SwAxis) == 88
+    # sw_source: 4 int32 + 6 float + pointer + int32 (+ padding)
+    assert C_sizeof(L.SwSource) == 4 * 4 + 6 * 4 + 8 + 8
+
+
+def C_sizeof(t):
+    import ctypes
+    return ctypes.sizeof(t)
+
+
+@pytest.mark.parametrize("order", range(1, 9))
+def test_product_coefficients_are_f32_of_oracle(order):
+    np.testing.assert_array_equal(diff_coff(order), O.diff_coff(order).astype(np.float32))
+
+
+def test_diff_coff_op_literals():
+    # the literal constants of src/Elastic2D_CPML.py:372-389
+    np.testing.assert_array_equal(diff_coff_op(2), np.float32([1.129042, -0.0430142]))
+    np.testing.assert_array_equal(diff_coff_op(5)[-1], np.float32(0.0007054313))
+    assert diff_coff_op(3).dtype == np.float32 and len(diff_coff_op(4)) == 4
+
+
+def test_pml_parameter_validation():
+    with pytest.raises(KeyError):
+        check_parameter({"vp_max": 1.0}, ("pml_x_thick",))
+    with pytest.raises(ValueError):
+        axis_pml(40, 2, 10.0, 1e-3, 0.0, 3000.0, 1e-6, 30.0, 2.0, 1.0, True, True)
+
+
+def test_memory_slab_whole_axis_when_pml_covers_it():
+    n = 12
+    ax = axis_pml(n, 2, 10.0, 1e-3, 60.0, 3000.0, 1e-6, 30.0, 2.0, 1.0, True, True)
+    assert ax.lo + (n - ax.hi) <= n
+    nontriv = (ax.a != 0) | (ax.k != 1) | (ax.a_h != 0) | (ax.k_h != 1)
+    assert not nontriv[ax.lo:ax.hi].any()
+
+
+def test_memory_slab_refuses_coefficients_outside_the_pml_test():
+    a = np.zeros(10, np.float32)
+    k = np.ones(10, np.float32)
+    a[5] = -0.1
+    inside = np.zeros(10, np.uint8)
+    with pytest.raises(ValueError):
+        memory_slab(a, k, a.copy(), k.copy(), inside, inside)
+
+
+def test_field_presents_reference_index_order():
+    ref = np.arange(2 * 3 * 4, dtype=np.float32).reshape(2, 3, 4)
+    t = to_device_layout(ref, "cpu")
+    assert t.shape == (4, 3, 2) and t.is_contiguous()
+    f = Field(t)
+    assert f.shape == (2, 3, 4)
+    np.testing.assert_array_equal(f.to_numpy(), ref)
+    assert f[1, 2, 3] == ref[1, 2, 3]
+    f[0, 1, 2] = -5.0
+    assert t[2, 1, 0].item() == -5.0
+    f.from_numpy(ref * 2)
+    np.testing.assert_array_equal(np.asarray(f), ref * 2)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_solver_refuses_to_run_without_gpu():
+    from seiswave_taichi_amd.Elastic3D_SSG_CPML import ElasticWAVE
+    v = np.ones((16, 16, 16), np.float32)
+    with pytest.raises(RuntimeError, match="GPU"):
+        ElasticWAVE(v, v, v, 1.0, 1.0, 1.0, 1e-4, 5, 5, 5, 6, 6, 6, 10, 2)
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "seiswave_taichi_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".hip", ".h", ".cpp")):
+                txt = open(os.path.join(dirpath, fn)).read()
+                assert "import oracle" not in txt and "from oracle" not in txt, fn
+                assert "seiswave_oracle" not in txt, fn
+
+
+def test_install_src_alias():
+    import sys
+    import seiswave_taichi_amd as pkg
+    saved = {k: sys.modules.pop(k) for k in list(sys.modules) if k == "src" or k.startswith("src.")}
+    try:
+        pkg.install_src_alias()
+        from src.Elastic3D_SSG_Tensor import ElasticWAVE  # noqa: F401
+        import src.Elastic2D_CPML as m
+        assert m.__name__ == "seiswave_taichi_amd.Elastic2D_CPML"
+    finally:
+        for k in [k for k in sys.modules if k == "src" or k.startswith("src.")]:
+            sys.modules.pop(k)
+        sys.modules.update(saved)
