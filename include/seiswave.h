@@ -174,6 +174,30 @@ int sw_ssg2d_free_surface(const sw_ssg2d *g, void *stream);
 /* 3-D variant (src/Elastic3D_SSG_Tensor.py:151-157): lam = 0, rho *= 0.5 on k < order. */
 int sw_ssg3d_free_surface_init(const sw_ssg3d *g, float *rho, float *lam, void *stream);
 
+/* ---- 2-D discrete adjoint (no counterpart in the reference; the gradient
+ * path of a seismogram misfit with respect to vp, vs, rho) ----
+ * Exact transpose of the SW_2D_VS step above, CPML memory variables included.
+ * Per backward step n: sw_ssg2d_adjoint_inject (receiver residuals of step n),
+ * sw_ssg2d_adjoint_stress with the forward state after the velocity stage of
+ * step n, then sw_ssg2d_adjoint_velocity with the forward state before it.
+ * The descriptors passed as `fwd` hold that forward state (and the model). */
+typedef struct sw_adj2d {
+    float *vx, *vz, *sxx, *szz, *sxz; /* adjoint wavefields [nz][nx]               */
+    float *mem;                       /* 10 adjoint memory variables, as sw_ssg2d  */
+    float *tmp;                       /* 4 scratch arrays [nz][nx]                 */
+    float *g_b1, *g_b4;               /* d misfit / d(dt/rho) at vx, vz cells      */
+    float *g_lamh, *g_muh, *g_muz;    /* d misfit / d(averaged lam, mu)            */
+} sw_adj2d;
+
+int sw_ssg2d_adjoint_stress(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream);
+int sw_ssg2d_adjoint_velocity(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream);
+/* adds residual[it] (an sw_receivers whose data holds residuals) at the receivers */
+int sw_ssg2d_adjoint_inject(const sw_ssg2d *g, const sw_adj2d *adj, const sw_receivers *residual,
+                            int32_t it, void *stream);
+/* gvp, gvs, grho [nz][nx] from the accumulated coefficient gradients */
+int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float *vp, const float *vs,
+                            float *gvp, float *gvs, float *grho, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,10 @@
  * (src/BaseFun.py:7,15); the kernel sees it as a constant of the field type. */
 #define OR_PI2 9.869604401089358
 
+/* Grids smaller than this run their loops on one thread (thread start-up and
+ * barrier costs dominate below it). */
+#define OR_PAR_MIN 200000L
+
 /* ---------------------------------------------------------------- stencils
  * src/Differential3D.py:7-44 / src/Differential2D.py:4-28.  Forward ("fm"):
  * sum_id c[id]*(f[i+id+1]-f[i-id]); backward ("bm"): c[id]*(f[i+id]-f[i-id-1]).
@@ -204,7 +208,7 @@ void or3d_cpml_velocity(or_state3d *st, int32_t tensor_order)
     const real *c = st->c;
     real **m = st->mem;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s + 1; k < nz - s; ++k) {
@@ -224,7 +228,7 @@ void or3d_cpml_velocity(or_state3d *st, int32_t tensor_order)
                 st->vx[p] += (dsxx + dsxzz + dsxyy) * dt / st->rho[p];
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s; j < ny - s - 1; ++j)
             for (int k = s + 1; k < nz - s; ++k) {
@@ -245,7 +249,7 @@ void or3d_cpml_velocity(or_state3d *st, int32_t tensor_order)
                 st->vy[p] += (dsxy + dsyy + dsyz) * dt / rho_half_y;
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s; k < nz - s - 1; ++k) {
@@ -282,7 +286,7 @@ void or3d_cpml_stress(or_state3d *st)
     const real *c = st->c;
     real **m = st->mem;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s + 1; k < nz - s; ++k) {
@@ -307,7 +311,7 @@ void or3d_cpml_stress(or_state3d *st)
                 st->szz[p] += (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt;
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s; k < nz - s; ++k) {
@@ -325,7 +329,7 @@ void or3d_cpml_stress(or_state3d *st)
                 st->sxy[p] += mu_a * (dvxy + dvyx) * dt;
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s; j < ny - s; ++j)
             for (int k = s; k < nz - s - 1; ++k) {
@@ -343,7 +347,7 @@ void or3d_cpml_stress(or_state3d *st)
                 st->sxz[p] += mu_a * (dvxz + dvzx) * dt;
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s; ++i)
         for (int j = s; j < ny - s - 1; ++j)
             for (int k = s; k < nz - s - 1; ++k) {
@@ -375,7 +379,7 @@ void or3d_spml_velocity(or_state3d *st)
     const real *c = st->c;
     real **q = st->split;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s + 1; k < nz - s; ++k) {
@@ -398,7 +402,7 @@ void or3d_spml_velocity(or_state3d *st)
                 }
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s; j < ny - s - 1; ++j)
             for (int k = s + 1; k < nz - s; ++k) {
@@ -421,7 +425,7 @@ void or3d_spml_velocity(or_state3d *st)
                 }
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s; k < nz - s - 1; ++k) {
@@ -456,7 +460,7 @@ void or3d_spml_stress(or_state3d *st)
     const real *c = st->c;
     real **q = st->split;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s + 1; k < nz - s; ++k) {
@@ -489,7 +493,7 @@ void or3d_spml_stress(or_state3d *st)
                 }
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < ny - s; ++j)
             for (int k = s; k < nz - s; ++k) {
@@ -509,7 +513,7 @@ void or3d_spml_stress(or_state3d *st)
                 }
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s; j < ny - s; ++j)
             for (int k = s; k < nz - s - 1; ++k) {
@@ -529,7 +533,7 @@ void or3d_spml_stress(or_state3d *st)
                 }
             }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * ny * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s; ++i)
         for (int j = s; j < ny - s - 1; ++j)
             for (int k = s; k < nz - s - 1; ++k) {
@@ -610,7 +614,7 @@ void or2d_vs_step(or_state2d *st, int32_t it, real amp)
     const real *c = st->c;
     real **m = st->mem;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < nz - s; ++j) {
             size_t p = I2(i, j);
@@ -625,7 +629,7 @@ void or2d_vs_step(or_state2d *st, int32_t it, real amp)
             }
             st->vx[p] += (dsxxdx + dsxzdz) * dt / st->rho[p];
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s; j < nz - s - 1; ++j) {
             size_t p = I2(i, j);
@@ -645,7 +649,7 @@ void or2d_vs_step(or_state2d *st, int32_t it, real amp)
         size_t p = I2(st->isx, st->isz);
         st->vz[p] += dt * amp / st->rho[p];
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s + 1; j < nz - s; ++j) {
             size_t p = I2(i, j);
@@ -664,7 +668,7 @@ void or2d_vs_step(or_state2d *st, int32_t it, real amp)
             st->sxx[p] += (lpm * dvxdx + lam_h * dvzdz) * dt;
             st->szz[p] += (lpm * dvzdz + lam_h * dvxdx) * dt;
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s; j < nz - s - 1; ++j) {
             size_t p = I2(i, j);
@@ -692,7 +696,7 @@ void or2d_sv_step(or_state2d *st, int32_t it, real amp)
     const real *c = st->c;
     real **m = st->mem;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s + 1; j < nz - s; ++j) {
             size_t p = I2(i, j);
@@ -708,7 +712,7 @@ void or2d_sv_step(or_state2d *st, int32_t it, real amp)
             }
             st->vx[p] += (dsxxdx + dsxzdz) * dt / rho_half_x;
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s; j < nz - s - 1; ++j) {
             size_t p = I2(i, j);
@@ -728,7 +732,7 @@ void or2d_sv_step(or_state2d *st, int32_t it, real amp)
         size_t p = I2(st->isx, st->isz);
         st->vx[p] += dt * amp / st->rho[p];
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < nz - s; ++j) {
             size_t p = I2(i, j);
@@ -746,7 +750,7 @@ void or2d_sv_step(or_state2d *st, int32_t it, real amp)
             st->sxx[p] += (lpm * dvxdx + lam * dvzdz) * dt;
             st->szz[p] += (lpm * dvzdz + lam * dvxdx) * dt;
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < nz - s; ++j) {
             size_t p = I2(i, j);
@@ -773,7 +777,7 @@ void or2d_rsg_step(or_state2d *st, real source)
     const real dx2 = (real)2 * dx, dz2 = (real)2 * dz;
     real **m = st->mem;
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s + 1; i < nx - s; ++i)
         for (int j = s + 1; j < nz - s; ++j) {
             size_t p = I2(i, j);
@@ -804,7 +808,7 @@ void or2d_rsg_step(or_state2d *st, real source)
             size_t p = I2(st->isx + a, st->isz + b);
             st->vz[p] += dt * source / ((real)8 * st->rho[p]);
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)nx * nz > OR_PAR_MIN)
     for (int i = s; i < nx - s - 1; ++i)
         for (int j = s; j < nz - s - 1; ++j) {
             size_t p = I2(i, j);

@@ -48,6 +48,11 @@ class SwSSG2D(C.Structure):
     )
 
 
+class SwAdj2D(C.Structure):
+    _fields_ = [(n, _fp) for n in ("vx", "vz", "sxx", "szz", "sxz", "mem", "tmp", "g_b1", "g_b4",
+                                   "g_lamh", "g_muh", "g_muz")]
+
+
 class SwSource(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
                 ("mt", C.c_float * 6), ("amp", _fp), ("namp", C.c_int32)]
@@ -61,6 +66,7 @@ class SwReceivers(C.Structure):
 # Every exported symbol of include/seiswave.h with its argument types.
 _P3, _P2 = C.POINTER(SwSSG3D), C.POINTER(SwSSG2D)
 _PS, _PR = C.POINTER(SwSource), C.POINTER(SwReceivers)
+_PA = C.POINTER(SwAdj2D)
 SIGNATURES = {
     "sw_version": [],
     "sw_last_error": [C.c_char_p, C.c_size_t],
@@ -80,6 +86,10 @@ SIGNATURES = {
     "sw_ssg2d_free_surface_init": [_P2, C.c_void_p],
     "sw_ssg2d_free_surface": [_P2, C.c_void_p],
     "sw_ssg3d_free_surface_init": [_P3, _fp, _fp, C.c_void_p],
+    "sw_ssg2d_adjoint_stress": [_P2, _PA, C.c_void_p],
+    "sw_ssg2d_adjoint_velocity": [_P2, _PA, C.c_void_p],
+    "sw_ssg2d_adjoint_inject": [_P2, _PA, _PR, C.c_int32, C.c_void_p],
+    "sw_ssg2d_model_gradient": [_P2, _PA, _fp, _fp, _fp, _fp, _fp, C.c_void_p],
 }
 
 _lib = None

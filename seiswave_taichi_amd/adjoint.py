@@ -1,0 +1,252 @@
+"""Seismogram modelling with an exact discrete adjoint (2-D VS scheme).
+
+``Elastic2DModeling`` runs the 2-D P-SV step of src/Elastic2D_CPML.py:127-200
+(update_SSG_VS) for a set of shots and records multi-component seismograms.
+``SeismogramFn`` is a ``torch.autograd.Function``: its forward returns the
+seismograms for (vp, vs, rho) and its backward runs the hand-written adjoint
+kernels of libseiswave (sw_ssg2d_adjoint_*), giving the gradient of any loss
+of the seismograms with respect to vp, vs and rho.
+
+Memory: the forward keeps a checkpoint of the full state every
+``checkpoint`` steps; the backward recomputes each segment forward from its
+checkpoint, keeping the state before and after the velocity stage of every
+step of the segment, then runs the adjoint backwards through it.
+
+Sources inject into the stresses (explosive or moment tensor), which do not
+depend on the model; force sources are refused because their 1/rho factor is
+not part of the adjoint.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._engine3d import gpu_device
+from .coeffs import diff_coff, ricker2
+from .pml import axis_pml, check_parameter
+
+NSTATE = 15  # vx vz sxx szz sxz + 10 memory variables
+_FIELD_ROW = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
+
+
+class Elastic2DModeling:
+    """Multi-shot 2-D VS modelling on one GPU.
+
+    shape (nx, nz); shots: list of (ix, iz) source cells; receivers: (N, 2)
+    cells; components: recorded fields.  ``source`` is "explosive" or a
+    moment tensor (Mxx, Mzz, Mxz).  ``wavelet`` defaults to a Ricker of
+    ``freq`` delayed by 1.5/freq.
+    """
+
+    def __init__(self, shape, dx, dz, dt, nt, accuracy, pml: dict, shots, receivers,
+                 pml_surface=(True, True, True, True), freq=10.0, components=("vx", "vz"),
+                 source="explosive", wavelet=None, checkpoint=50):
+        check_parameter(pml, ("pml_x_thick", "pml_z_thick"))
+        self.lib = L.load()
+        self.device = gpu_device()
+        self.nx, self.nz = (int(v) for v in shape)
+        self.dx, self.dz, self.dt, self.nt, self.order = dx, dz, dt, int(nt), int(accuracy)
+        self.c = diff_coff(self.order)
+        self.checkpoint = int(checkpoint)
+        dev = self.device
+        ax = axis_pml(self.nx, self.order, dx, dt, pml["pml_x_thick"], pml["vp_max"], pml["Rcoef"],
+                      pml["alpha_max_pml"], pml["kmax_pml"], pml["theta"], pml_surface[0], pml_surface[1])
+        az = axis_pml(self.nz, self.order, dz, dt, pml["pml_z_thick"], pml["vp_max"], pml["Rcoef"],
+                      pml["alpha_max_pml"], pml["kmax_pml"], pml["theta"], pml_surface[2], pml_surface[3])
+        self._axes = []
+        for a in (ax, az):
+            coeff = torch.from_numpy(np.stack([a.a, a.b, a.k, a.a_h, a.b_h, a.k_h, a.d, a.d_h])).to(dev).contiguous()
+            flags = torch.from_numpy(np.stack([a.inside, a.inside_h])).to(dev).contiguous()
+            self._axes.append((coeff, flags))
+        self.shots = [tuple(int(v) for v in s) for s in shots]
+        for s in self.shots:
+            if not (1 <= s[0] < self.nx - 1 and 1 <= s[1] < self.nz - 1):
+                raise IndexError(f"shot {s} outside the grid")
+        rec = np.asarray(receivers, dtype=np.int64).reshape(-1, 2)
+        if rec.size and (rec[:, 0].min() < 0 or rec[:, 0].max() >= self.nx or rec[:, 1].min() < 0
+                         or rec[:, 1].max() >= self.nz):
+            raise IndexError("receiver outside the grid")
+        self.rec_idx = torch.from_numpy((rec[:, 1] * self.nx + rec[:, 0]).astype(np.int32)).to(dev)
+        self.nrec = len(rec)
+        self.components = tuple(components)
+        for cn in self.components:
+            if cn not in _FIELD_ROW:
+                raise ValueError(f"unknown component {cn}")
+        if isinstance(source, str):
+            if source != "explosive":
+                raise NotImplementedError("the adjoint covers stress sources (explosive or moment tensor)")
+            self.src_kind, mt = L.SW_SRC_EXPLOSIVE, np.zeros(6, np.float32)
+        else:
+            mxx, mzz, mxz = (float(v) for v in source)
+            self.src_kind, mt = L.SW_SRC_MOMENT, np.array([mxx, 0, mzz, 0, mxz, 0], np.float32)
+        self.mt = mt
+        w = ricker2(np.arange(self.nt), dt, 1.5 / freq, freq, 1.0) if wavelet is None else wavelet
+        self.amp = torch.from_numpy(np.ascontiguousarray(w, dtype=np.float32)).to(dev)
+        # model-dependent device arrays are set by set_model
+        self.rho = self.lam = self.mu = None
+
+    # -------------------------------------------------------------- plumbing
+    def set_model(self, vp: torch.Tensor, vs: torch.Tensor, rho: torch.Tensor):
+        """vp, vs, rho: (nx, nz) tensors in reference index order."""
+        dl = lambda t: t.detach().to(self.device, torch.float32).t().contiguous()  # noqa: E731
+        self.vp_d, self.vs_d, self.rho = dl(vp), dl(vs), dl(rho)
+        vs2 = self.vs_d * self.vs_d
+        self.mu = self.rho * vs2
+        self.lam = self.rho * (self.vp_d * self.vp_d - 2 * vs2)
+
+    def new_state(self, n=None):
+        shape = (NSTATE, self.nz, self.nx) if n is None else (n, NSTATE, self.nz, self.nx)
+        return torch.zeros(shape, dtype=torch.float32, device=self.device)
+
+    def desc(self, state: torch.Tensor) -> L.SwSSG2D:
+        g = L.SwSSG2D()
+        g.nx, g.nz, g.order, g.scheme = self.nx, self.nz, self.order, L.SW_2D_VS
+        g.dx, g.dz, g.dt = float(self.dx), float(self.dz), float(self.dt)
+        for i, v in enumerate(self.c):
+            g.c[i] = float(v)
+        g.rho, g.lam, g.mu = L.fptr(self.rho), L.fptr(self.lam), L.fptr(self.mu)
+        base = state.data_ptr()
+        plane = 4 * self.nx * self.nz
+        for name, row in _FIELD_ROW.items():
+            setattr(g, name, C.cast(C.c_void_p(base + row * plane), L._fp))
+        g.mem = C.cast(C.c_void_p(base + 5 * plane), L._fp)
+        for name, (coeff, flags), n in zip(("px", "pz"), self._axes, (self.nx, self.nz)):
+            a = L.SwAxis()
+            cb = coeff.data_ptr()
+            for r, fname in enumerate(("a", "b", "k", "a_h", "b_h", "k_h", "d", "d_h")):
+                setattr(a, fname, C.cast(C.c_void_p(cb + 4 * r * n), L._fp))
+            a.in_ = C.cast(C.c_void_p(flags.data_ptr()), L._u8p)
+            a.in_h = C.cast(C.c_void_p(flags.data_ptr() + n), L._u8p)
+            a.lo, a.hi = 0, n
+            setattr(g, name, a)
+        return g
+
+    def source(self, shot) -> L.SwSource:
+        s = L.SwSource()
+        s.kind, s.ix, s.iy, s.iz = self.src_kind, shot[0], 0, shot[1]
+        for i, v in enumerate(self.mt):
+            s.mt[i] = float(v)
+        s.amp, s.namp = L.fptr(self.amp), self.nt
+        return s
+
+    def receivers(self, data: torch.Tensor) -> L.SwReceivers:
+        r = L.SwReceivers()
+        r.nrec, r.ncomp, r.nt = self.nrec, len(self.components), self.nt
+        for i, cn in enumerate(self.components):
+            r.comp[i] = L.FIELD_CODE[cn]
+        r.idx, r.data = L.i32ptr(self.rec_idx), L.fptr(data)
+        return r
+
+    def stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    # --------------------------------------------------------------- forward
+    def forward_shot(self, shot, data: torch.Tensor, checkpoints: torch.Tensor | None = None):
+        """Run one shot from a zero state; record into data (nt, ncomp, nrec);
+        store the state before steps 0, K, 2K, ... in ``checkpoints``."""
+        lib, st = self.lib, self.stream()
+        state = self.new_state()
+        g = self.desc(state)
+        src, rec = self.source(shot), self.receivers(data)
+        K = self.checkpoint
+        for s0 in range(0, self.nt, K):
+            if checkpoints is not None:
+                checkpoints[s0 // K].copy_(state)
+            n = min(K, self.nt - s0)
+            L.check(lib.sw_ssg2d_run(g, src, rec, s0, n, st), "sw_ssg2d_run")
+        return state
+
+    def model(self, vp, vs, rho, keep_checkpoints=False):
+        """Seismograms (nshot, nt, ncomp, nrec) and, optionally, the checkpoints."""
+        self.set_model(vp, vs, rho)
+        nck = (self.nt + self.checkpoint - 1) // self.checkpoint
+        data = torch.zeros((len(self.shots), self.nt, len(self.components), self.nrec), dtype=torch.float32,
+                           device=self.device)
+        cks = []
+        for s, shot in enumerate(self.shots):
+            ck = self.new_state(nck) if keep_checkpoints else None
+            self.forward_shot(shot, data[s], ck)
+            cks.append(ck)
+        return data, cks
+
+    # -------------------------------------------------------------- backward
+    def gradient(self, grad_data: torch.Tensor, checkpoints):
+        """d loss / d (vp, vs, rho) for d loss / d data = grad_data, as (nx, nz)
+        tensors in reference order.  Requires the model set by ``model``."""
+        lib, st = self.lib, self.stream()
+        nz, nx, K = self.nz, self.nx, self.checkpoint
+        acc = torch.zeros((5, nz, nx), dtype=torch.float32, device=self.device)  # g_b1 g_b4 g_lamh g_muh g_muz
+        adj = self.new_state()
+        tmp = torch.zeros((4, nz, nx), dtype=torch.float32, device=self.device)
+        pre = self.new_state(K)
+        mid = self.new_state(K)
+        work = self.new_state()
+        q = self._adj_desc(adj, tmp, acc)
+        gw = self.desc(work)
+        gpre = [self.desc(pre[t]) for t in range(K)]
+        gmid = [self.desc(mid[t]) for t in range(K)]
+        plane = nx * nz
+        for s, shot in enumerate(self.shots):
+            res = grad_data[s].detach().to(torch.float32).contiguous()
+            rres = self.receivers(res)
+            src = self.source(shot)
+            adj.zero_()
+            for s0 in reversed(range(0, self.nt, K)):
+                n = min(K, self.nt - s0)
+                work.copy_(checkpoints[s][s0 // K])
+                for t in range(n):
+                    it = s0 + t
+                    L.check(lib.sw_ssg2d_inject(gw, src, it, st), "sw_ssg2d_inject")
+                    pre[t].copy_(work)
+                    L.check(lib.sw_ssg2d_velocity(gw, st), "sw_ssg2d_velocity")
+                    mid[t].copy_(work)
+                    L.check(lib.sw_ssg2d_stress(gw, st), "sw_ssg2d_stress")
+                for t in reversed(range(n)):
+                    it = s0 + t
+                    L.check(lib.sw_ssg2d_adjoint_inject(gw, q, rres, it, st), "sw_ssg2d_adjoint_inject")
+                    L.check(lib.sw_ssg2d_adjoint_stress(gmid[t], q, st), "sw_ssg2d_adjoint_stress")
+                    L.check(lib.sw_ssg2d_adjoint_velocity(gpre[t], q, st), "sw_ssg2d_adjoint_velocity")
+        out = torch.empty((3, nz, nx), dtype=torch.float32, device=self.device)
+        L.check(lib.sw_ssg2d_model_gradient(self.desc(work), q, L.fptr(self.vp_d), L.fptr(self.vs_d),
+                                            L.fptr(out[0]), L.fptr(out[1]), L.fptr(out[2]), st),
+                "sw_ssg2d_model_gradient")
+        del plane
+        return tuple(out[i].t() for i in range(3))
+
+    def _adj_desc(self, adj, tmp, acc) -> "L.SwAdj2D":
+        q = L.SwAdj2D()
+        plane = 4 * self.nx * self.nz
+        base = adj.data_ptr()
+        for name, row in _FIELD_ROW.items():
+            setattr(q, name, C.cast(C.c_void_p(base + row * plane), L._fp))
+        q.mem = C.cast(C.c_void_p(base + 5 * plane), L._fp)
+        q.tmp = L.fptr(tmp)
+        for r, name in enumerate(("g_b1", "g_b4", "g_lamh", "g_muh", "g_muz")):
+            setattr(q, name, C.cast(C.c_void_p(acc.data_ptr() + r * plane), L._fp))
+        return q
+
+
+class SeismogramFn(torch.autograd.Function):
+    """data = F(vp, vs, rho); backward through the hand-written adjoint."""
+
+    @staticmethod
+    def forward(ctx, vp, vs, rho, modeling: Elastic2DModeling):
+        data, cks = modeling.model(vp, vs, rho, keep_checkpoints=True)
+        ctx.modeling = modeling
+        ctx.cks = cks
+        ctx.model = (vp.detach(), vs.detach(), rho.detach())
+        return data
+
+    @staticmethod
+    def backward(ctx, grad_data):
+        m = ctx.modeling
+        m.set_model(*ctx.model)
+        gvp, gvs, grho = m.gradient(grad_data, ctx.cks)
+        return gvp, gvs, grho, None
+
+
+def seismograms(vp, vs, rho, modeling: Elastic2DModeling):
+    return SeismogramFn.apply(vp, vs, rho, modeling)

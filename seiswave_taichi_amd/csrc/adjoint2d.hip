@@ -1,0 +1,314 @@
+// Exact discrete adjoint of the 2-D VS step (src/Elastic2D_CPML.py:127-200)
+// with CPML memory variables, and the model gradient it yields.
+//
+// The forward step, as a map of the state u = (vx, vz, sxx, szz, sxz, m0..m9):
+//   V (velocity stage):  vx += (Dx' sxx + Dz' sxz) * dt/rho          (vx cells)
+//                        vz += (Dz' szz + Dx' sxz) * dt/rho4         (vz cells)
+//   S (stress stage):    sxx += (lpm Dx' vx + lamh Dz' vz) dt,
+//                        szz += (lpm Dz' vz + lamh Dx' vx) dt         (sxx cells)
+//                        sxz += muz (Dz' vx + Dx' vz) dt              (sxz cells)
+// where D' = D/kappa + m_new, m_new = b m + a D inside the PML test and D' = D
+// outside.  The adjoint runs S^T then V^T per step, backwards in time.  Each
+// transpose is two kernels: pass A evaluates, per output cell, the adjoint of
+// its pre-CPML derivatives (and updates the memory-variable adjoints and the
+// model-coefficient gradients, which need the forward derivatives recomputed
+// from the stored forward state); pass B gathers those into the adjoint of the
+// differentiated fields, the transpose of the staggered stencils.
+#include "sw_internal.h"
+
+namespace sw {
+
+namespace {
+
+// Adjoint of one CPML-corrected derivative.  Forward: m1 = b m + a D,
+// D' = D/k + m1.  Given Dbar' (adjoint of D') and mbar (adjoint of m1 from
+// later steps), returns Dbar and replaces mbar by the adjoint of m.
+__device__ __forceinline__ float cpml_adjoint(float dbar_p, float *mbar, float a, float b, float k)
+{
+    const float mt = *mbar + dbar_p;
+    *mbar = b * mt;
+    return dbar_p / k + a * mt;
+}
+
+__device__ __forceinline__ float fwd_cpml(float D, float m, float a, float b, float k)
+{
+    return D / k + (b * m + a * D);
+}
+
+// ------------------------------------------------------------- stress stage
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = f.nx, nz = f.nz;
+    if (i >= nx || j >= nz) return;
+    const long SZ = nx, NC = (long)nx * nz;
+    const long p = (long)j * SZ + i;
+    const float dx = f.dx, dz = f.dz, dt = f.dt;
+    const float *c = f.c;
+    const sw_axis &X = f.px, &Z = f.pz;
+    float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+    // sxx, szz at (i+1/2, j): forward src/Elastic2D_CPML.py:172-186
+    if (i >= N && i < nx - N - 1 && j >= N + 1 && j < nz - N) {
+        const float gxx = q.sxx[p] * dt, gzz = q.szz[p] * dt;
+        const float lam_h = 0.5f * (f.lam[p] + f.lam[p + 1]);
+        const float mu_h = 0.5f * (f.mu[p] + f.mu[p + 1]);
+        const float lpm = lam_h + 2.f * mu_h;
+        float Dx = d_fm<N>(f.vx, p, 1, c) / dx;
+        float Dz = d_bm<N>(f.vz, p, SZ, c) / dz;
+        const bool in = X.in_h[i] | Z.in[j];
+        float Dxp = Dx, Dzp = Dz;
+        if (in) {
+            Dxp = fwd_cpml(Dx, f.mem[6 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
+            Dzp = fwd_cpml(Dz, f.mem[9 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
+        }
+        q.g_lamh[p] += (gxx + gzz) * (Dxp + Dzp);
+        q.g_muh[p] += 2.f * (gxx * Dxp + gzz * Dzp);
+        float bx = lpm * gxx + lam_h * gzz;
+        float bz = lam_h * gxx + lpm * gzz;
+        if (in) {
+            bx = cpml_adjoint(bx, q.mem + 6 * NC + p, X.a_h[i], X.b_h[i], X.k_h[i]);
+            bz = cpml_adjoint(bz, q.mem + 9 * NC + p, Z.a[j], Z.b[j], Z.k[j]);
+        }
+        t0 = bx / dx;
+        t1 = bz / dz;
+    }
+    // sxz at (i, j+1/2): :188-199
+    if (i >= N + 1 && i < nx - N && j >= N && j < nz - N - 1) {
+        const float g = q.sxz[p] * dt;
+        const float mu_z = 0.5f * (f.mu[p + SZ] + f.mu[p]);
+        float Dx = d_bm<N>(f.vz, p, 1, c) / dx;
+        float Dz = d_fm<N>(f.vx, p, SZ, c) / dz;
+        const bool in = X.in[i] | Z.in_h[j];
+        float Dxp = Dx, Dzp = Dz;
+        if (in) {
+            Dxp = fwd_cpml(Dx, f.mem[8 * NC + p], X.a[i], X.b[i], X.k[i]);
+            Dzp = fwd_cpml(Dz, f.mem[7 * NC + p], Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+        }
+        q.g_muz[p] += g * (Dzp + Dxp);
+        float bx = mu_z * g, bz = mu_z * g;
+        if (in) {
+            bx = cpml_adjoint(bx, q.mem + 8 * NC + p, X.a[i], X.b[i], X.k[i]);
+            bz = cpml_adjoint(bz, q.mem + 7 * NC + p, Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+        }
+        t2 = bx / dx;
+        t3 = bz / dz;
+    }
+    q.tmp[0 * NC + p] = t0;
+    q.tmp[1 * NC + p] = t1;
+    q.tmp[2 * NC + p] = t2;
+    q.tmp[3 * NC + p] = t3;
+}
+
+__device__ __forceinline__ float tap(const float *t, long idx, long n)
+{
+    return (idx >= 0 && idx < n) ? t[idx] : 0.f;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = f.nx, nz = f.nz;
+    if (i >= nx || j >= nz) return;
+    const long SZ = nx, NC = (long)nx * nz;
+    const long p = (long)j * SZ + i;
+    const float *c = f.c;
+    const float *T0 = q.tmp, *T1 = q.tmp + NC, *T2 = q.tmp + 2 * NC, *T3 = q.tmp + 3 * NC;
+    float ax = 0.f, az = 0.f;
+#pragma unroll
+    for (int id = 0; id < N; ++id) {
+        // sxx cells read vx[p+id+1] (+) and vx[p-id] (-) along x (d_fm)
+        ax += c[id] * (((i - id - 1) >= 0 ? T0[p - id - 1] : 0.f) - ((i + id) < nx ? T0[p + id] : 0.f));
+        // sxz cells read vx[p+(id+1)SZ] (+) and vx[p-id SZ] (-) along z (d_fm)
+        ax += c[id] * (tap(T3, p - (id + 1) * SZ, NC) - tap(T3, p + id * SZ, NC));
+        // sxx cells read vz[p+id SZ] (+) and vz[p-(id+1)SZ] (-) along z (d_bm)
+        az += c[id] * (tap(T1, p - id * SZ, NC) - tap(T1, p + (id + 1) * SZ, NC));
+        // sxz cells read vz[p+id] (+) and vz[p-id-1] (-) along x (d_bm)
+        az += c[id] * (((i - id) >= 0 ? T2[p - id] : 0.f) - ((i + id + 1) < nx ? T2[p + id + 1] : 0.f));
+    }
+    q.vx[p] += ax;
+    q.vz[p] += az;
+}
+
+// ----------------------------------------------------------- velocity stage
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = f.nx, nz = f.nz;
+    if (i >= nx || j >= nz) return;
+    const long SZ = nx, NC = (long)nx * nz;
+    const long p = (long)j * SZ + i;
+    const float dx = f.dx, dz = f.dz, dt = f.dt;
+    const float *c = f.c;
+    const sw_axis &X = f.px, &Z = f.pz;
+    float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+    // vx at (i, j): src/Elastic2D_CPML.py:144-154
+    if (i >= N + 1 && i < nx - N && j >= N + 1 && j < nz - N) {
+        const float vb = q.vx[p];
+        const float g = vb * dt / f.rho[p];
+        float Dx = d_bm<N>(f.sxx, p, 1, c) / dx;
+        float Dz = d_bm<N>(f.sxz, p, SZ, c) / dz;
+        const bool in = X.in[i] | Z.in[j];
+        float Dxp = Dx, Dzp = Dz;
+        if (in) {
+            Dxp = fwd_cpml(Dx, f.mem[0 * NC + p], X.a[i], X.b[i], X.k[i]);
+            Dzp = fwd_cpml(Dz, f.mem[3 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
+        }
+        q.g_b1[p] += vb * (Dxp + Dzp);
+        float bx = g, bz = g;
+        if (in) {
+            bx = cpml_adjoint(bx, q.mem + 0 * NC + p, X.a[i], X.b[i], X.k[i]);
+            bz = cpml_adjoint(bz, q.mem + 3 * NC + p, Z.a[j], Z.b[j], Z.k[j]);
+        }
+        t0 = bx / dx;
+        t1 = bz / dz;
+    }
+    // vz at (i+1/2, j+1/2): :156-167
+    if (i >= N && i < nx - N - 1 && j >= N && j < nz - N - 1) {
+        const float vb = q.vz[p];
+        const float rho4 = 0.25f * (f.rho[p] + f.rho[p + SZ] + f.rho[p + 1 + SZ] + f.rho[p + 1]);
+        const float g = vb * dt / rho4;
+        float Dx = d_fm<N>(f.sxz, p, 1, c) / dx;
+        float Dz = d_fm<N>(f.szz, p, SZ, c) / dz;
+        const bool in = X.in_h[i] | Z.in_h[j];
+        float Dxp = Dx, Dzp = Dz;
+        if (in) {
+            Dxp = fwd_cpml(Dx, f.mem[2 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
+            Dzp = fwd_cpml(Dz, f.mem[5 * NC + p], Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+        }
+        q.g_b4[p] += vb * (Dzp + Dxp);
+        float bx = g, bz = g;
+        if (in) {
+            bx = cpml_adjoint(bx, q.mem + 2 * NC + p, X.a_h[i], X.b_h[i], X.k_h[i]);
+            bz = cpml_adjoint(bz, q.mem + 5 * NC + p, Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+        }
+        t2 = bx / dx;
+        t3 = bz / dz;
+    }
+    q.tmp[0 * NC + p] = t0;
+    q.tmp[1 * NC + p] = t1;
+    q.tmp[2 * NC + p] = t2;
+    q.tmp[3 * NC + p] = t3;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = f.nx, nz = f.nz;
+    if (i >= nx || j >= nz) return;
+    const long SZ = nx, NC = (long)nx * nz;
+    const long p = (long)j * SZ + i;
+    const float *c = f.c;
+    const float *T0 = q.tmp, *T1 = q.tmp + NC, *T2 = q.tmp + 2 * NC, *T3 = q.tmp + 3 * NC;
+    float axx = 0.f, axz = 0.f, azz = 0.f;
+#pragma unroll
+    for (int id = 0; id < N; ++id) {
+        // vx cells read sxx[p+id] (+), sxx[p-id-1] (-) along x (d_bm)
+        axx += c[id] * (((i - id) >= 0 ? T0[p - id] : 0.f) - ((i + id + 1) < nx ? T0[p + id + 1] : 0.f));
+        // vx cells read sxz[p+id SZ] (+), sxz[p-(id+1)SZ] (-) along z (d_bm)
+        axz += c[id] * (tap(T1, p - id * SZ, NC) - tap(T1, p + (id + 1) * SZ, NC));
+        // vz cells read sxz[p+id+1] (+), sxz[p-id] (-) along x (d_fm)
+        axz += c[id] * (((i - id - 1) >= 0 ? T2[p - id - 1] : 0.f) - ((i + id) < nx ? T2[p + id] : 0.f));
+        // vz cells read szz[p+(id+1)SZ] (+), szz[p-id SZ] (-) along z (d_fm)
+        azz += c[id] * (tap(T3, p - (id + 1) * SZ, NC) - tap(T3, p + id * SZ, NC));
+    }
+    q.sxx[p] += axx;
+    q.sxz[p] += axz;
+    q.szz[p] += azz;
+}
+
+// Residual injection: adjoint of the receiver gather data[it] = field[idx].
+__global__ void k_adj_inject(sw_adj2d q, sw_receivers r, int it)
+{
+    const int comp = threadIdx.x;
+    if (comp >= r.ncomp) return;
+    float *fld = r.comp[comp] == 0 ? q.vx : (r.comp[comp] == 2 ? q.vz : (r.comp[comp] == 3 ? q.sxx :
+                 (r.comp[comp] == 5 ? q.szz : q.sxz)));
+    const float *res = r.data + ((long)it * r.ncomp + comp) * r.nrec;
+    for (int k = 0; k < r.nrec; ++k) fld[r.idx[k]] += res[k];  // serial: duplicate receivers add up
+}
+
+// Model gradient from the accumulated coefficient gradients:
+//   lam_h = (lam[p]+lam[p+1])/2, mu_h likewise (sxx cells); mu_z = (mu[p+SZ]+mu[p])/2
+//   (sxz cells); b1 = dt/rho (vx cells); b4 = dt/rho4 (vz cells); then
+//   mu = rho vs^2, lam = rho (vp^2 - 2 vs^2).
+__global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const float *vs, float *gvp,
+                                 float *gvs, float *grho)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = f.nx, nz = f.nz;
+    if (i >= nx || j >= nz) return;
+    const long SZ = nx;
+    const long p = (long)j * SZ + i;
+    const float dt = f.dt;
+    float gl = 0.5f * q.g_lamh[p] + (i >= 1 ? 0.5f * q.g_lamh[p - 1] : 0.f);
+    float gm = 0.5f * q.g_muh[p] + (i >= 1 ? 0.5f * q.g_muh[p - 1] : 0.f);
+    gm += 0.5f * q.g_muz[p] + (j >= 1 ? 0.5f * q.g_muz[p - SZ] : 0.f);
+    const float r = f.rho[p];
+    float gr = q.g_b1[p] * (-dt / (r * r));
+    // vz cell p' uses rho4 over (p', p'+SZ, p'+1+SZ, p'+1): cell p is in the stencils
+    // of p, p-SZ, p-1-SZ, p-1
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int ii = i - a, jj = j - b;
+            if (ii < 0 || jj < 0 || ii >= nx - 1 || jj >= nz - 1) continue;
+            const long pp = (long)jj * SZ + ii;
+            const float r4 = 0.25f * (f.rho[pp] + f.rho[pp + SZ] + f.rho[pp + 1 + SZ] + f.rho[pp + 1]);
+            gr += 0.25f * q.g_b4[pp] * (-dt / (r4 * r4));
+        }
+    const float a = vp[p], s = vs[p];
+    gvp[p] = gl * 2.f * r * a;
+    gvs[p] = gl * (-4.f * r * s) + gm * 2.f * r * s;
+    grho[p] = gr + gl * (a * a - 2.f * s * s) + gm * s * s;
+}
+
+inline dim3 grid2(const sw_ssg2d &g, dim3 blk)
+{
+    return dim3((g.nx + blk.x - 1) / blk.x, (g.nz + blk.y - 1) / blk.y, 1);
+}
+
+}  // namespace
+
+int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
+{
+    const dim3 blk(64, 4, 1);
+    SW_DISPATCH_ORDER(f->order, {
+        hipLaunchKernelGGL((k_adj_stress_a<N>), grid2(*f, blk), blk, 0, st, *f, *q);
+        hipLaunchKernelGGL((k_adj_stress_b<N>), grid2(*f, blk), blk, 0, st, *f, *q);
+    });
+    return launch_status("2-D adjoint stress stage");
+}
+
+int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
+{
+    const dim3 blk(64, 4, 1);
+    SW_DISPATCH_ORDER(f->order, {
+        hipLaunchKernelGGL((k_adj_velocity_a<N>), grid2(*f, blk), blk, 0, st, *f, *q);
+        hipLaunchKernelGGL((k_adj_velocity_b<N>), grid2(*f, blk), blk, 0, st, *f, *q);
+    });
+    return launch_status("2-D adjoint velocity stage");
+}
+
+int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(16), 0, st, *q, *r, it);
+    return launch_status("2-D adjoint residual injection");
+}
+
+int launch_adj2d_model_gradient(const sw_ssg2d *f, const sw_adj2d *q, const float *vp, const float *vs, float *gvp,
+                                float *gvs, float *grho, hipStream_t st)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_model_gradient, grid2(*f, blk), blk, 0, st, *f, *q, vp, vs, gvp, gvs, grho);
+    return launch_status("2-D model gradient");
+}
+
+}  // namespace sw

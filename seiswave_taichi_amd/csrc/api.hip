@@ -24,6 +24,11 @@ int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st);
 int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st);
+int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
+int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
+int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st);
+int launch_adj2d_model_gradient(const sw_ssg2d *f, const sw_adj2d *q, const float *vp, const float *vs, float *gvp,
+                                float *gvs, float *grho, hipStream_t st);
 
 static int check_axis(const sw_axis &a, int n, const char *name, bool need_cpml, bool need_split)
 {
@@ -414,6 +419,54 @@ int sw_ssg3d_free_surface_init(const sw_ssg3d *g, float *rho, float *lam, void *
     hipLaunchKernelGGL(k_free_surface_init3d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream),
                        g->nx, g->ny, g->order, rho, lam);
     return launch_status("3-D free-surface init");
+}
+
+static int check_adj2d(const sw_ssg2d *g, const sw_adj2d *q)
+{
+    int rc = check_ssg2d(g);
+    if (rc) return rc;
+    SW_REQUIRE(g->scheme == SW_2D_VS, "the 2-D adjoint covers the VS scheme");
+    SW_REQUIRE(q && q->vx && q->vz && q->sxx && q->szz && q->sxz && q->mem && q->tmp && q->g_b1 && q->g_b4 &&
+                   q->g_lamh && q->g_muh && q->g_muz,
+               "adjoint arrays missing");
+    return SW_OK;
+}
+
+int sw_ssg2d_adjoint_stress(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream)
+{
+    int rc = check_adj2d(fwd, adj);
+    if (rc) return rc;
+    return launch_adj2d_stress(fwd, adj, as_stream(stream));
+}
+
+int sw_ssg2d_adjoint_velocity(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream)
+{
+    int rc = check_adj2d(fwd, adj);
+    if (rc) return rc;
+    return launch_adj2d_velocity(fwd, adj, as_stream(stream));
+}
+
+int sw_ssg2d_adjoint_inject(const sw_ssg2d *g, const sw_adj2d *adj, const sw_receivers *residual, int32_t it,
+                            void *stream)
+{
+    int rc = check_adj2d(g, adj);
+    if (rc) return rc;
+    if ((rc = check_receivers(residual, it))) return rc;
+    SW_REQUIRE(residual->ncomp <= 16, "too many residual components");
+    for (int c = 0; c < residual->ncomp; ++c)
+        SW_REQUIRE(fields2d(g).f[residual->comp[c]] != nullptr, "component %d does not exist in 2-D",
+                   residual->comp[c]);
+    if (residual->nrec == 0) return SW_OK;
+    return launch_adj2d_inject(adj, residual, it, as_stream(stream));
+}
+
+int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float *vp, const float *vs, float *gvp,
+                            float *gvs, float *grho, void *stream)
+{
+    int rc = check_adj2d(g, adj);
+    if (rc) return rc;
+    SW_REQUIRE(vp && vs && gvp && gvs && grho, "model gradient arrays missing");
+    return launch_adj2d_model_gradient(g, adj, vp, vs, gvp, gvs, grho, as_stream(stream));
 }
 
 }  // extern "C"

@@ -16,3 +16,8 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:  # pragma: no cover
         return False
+
+
+# The oracle's OpenMP threads must not spin-wait when a host runs more
+# threads than it has cores (read once, when the oracle library loads).
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")

@@ -1,0 +1,124 @@
+"""The hand-written 2-D adjoint: transpose identities of each stage and the
+model gradient against central finite differences of the f64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.cases import layered_model, pml_dict
+
+pytestmark = pytest.mark.gpu
+
+
+def _modeling(shape=(60, 48), nt=160, order=2, shots=((30, 6),), ck=40):
+    from seiswave_taichi_amd.adjoint import Elastic2DModeling
+    d = (10.0, 10.0)
+    rx = np.arange(8, shape[0] - 8, 4)
+    rec = np.stack([rx, np.full_like(rx, 5)], 1)
+    p = pml_dict(4700.0, 8, d, 20.0, 2700.0, dims=2)
+    m = Elastic2DModeling(shape, *d, 8e-4, nt, order, p, shots, rec, freq=20.0, checkpoint=ck)
+    return m, p, rec
+
+
+def _rand_state(m, seed):
+    g = torch.Generator(device=m.device).manual_seed(seed)
+    x = torch.randn((15, m.nz, m.nx), generator=g, device=m.device, dtype=torch.float32)
+    return x
+
+
+@pytest.mark.parametrize("stage", ["stress", "velocity"])
+def test_stage_transpose_dot_product(stage):
+    """<y, S x> = <S^T y, x> for each stage map S (state incl. memory variables)."""
+    from seiswave_taichi_amd import _lib as L
+    m, _, _ = _modeling()
+    vs, vp, rho = layered_model((m.nx, m.nz), seed=1)
+    m.set_model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
+    lib, st = m.lib, m.stream()
+    x = _rand_state(m, 1)
+    y = _rand_state(m, 2)
+    sx = x.clone()
+    gx = m.desc(sx)
+    fn = lib.sw_ssg2d_stress if stage == "stress" else lib.sw_ssg2d_velocity
+    L.check(fn(gx, st), stage)
+    adj = y.clone()
+    tmp = torch.zeros((4, m.nz, m.nx), device=m.device)
+    acc = torch.zeros((5, m.nz, m.nx), device=m.device)
+    q = m._adj_desc(adj, tmp, acc)
+    fwd = m.desc(x.clone())
+    afn = lib.sw_ssg2d_adjoint_stress if stage == "stress" else lib.sw_ssg2d_adjoint_velocity
+    L.check(afn(fwd, q, st), "adjoint " + stage)
+    torch.cuda.synchronize()
+    lhs = torch.sum(y.double() * sx.double()).item()
+    rhs = torch.sum(adj.double() * x.double()).item()
+    assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), 1.0), (lhs, rhs)
+
+
+def _oracle_misfit(vp, vs, rho, m, rec, p, dobs, shots):
+    """f64 oracle forward with explosive pre-injection; returns (J, data)."""
+    total = 0.0
+    datas = []
+    amps = m.amp.cpu().numpy().astype(np.float64)
+    for s, (ix, iz) in enumerate(shots):
+        o = O.Oracle2D(vs, vp, rho, m.dx, m.dz, m.dt, ix, iz, 0, 0, m.nt, m.order, freq=20.0, precision="f64")
+        o.set_pml([True] * 4, p)
+        data = np.zeros((m.nt, 2, len(rec)))
+        for it in range(m.nt):
+            e = amps[it] * (-0.70710678118654752)
+            o.fields["sxx"][ix, iz] += e
+            o.fields["szz"][ix, iz] += e
+            o.vs_step(it, 0.0)
+            data[it, 0] = o.fields["vx"][rec[:, 0], rec[:, 1]]
+            data[it, 1] = o.fields["vz"][rec[:, 0], rec[:, 1]]
+        datas.append(data)
+        if dobs is not None:
+            total += 0.5 * float(np.sum((data - dobs[s]) ** 2))
+    return total, np.stack(datas)
+
+
+def test_forward_modeling_matches_oracle():
+    m, p, rec = _modeling()
+    vs, vp, rho = layered_model((m.nx, m.nz), seed=3)
+    data, _ = m.model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
+    _, ref = _oracle_misfit(vp.astype(np.float64), vs.astype(np.float64), rho.astype(np.float64), m, rec, p,
+                            None, m.shots)
+    got = data.cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-5, err
+
+
+def test_gradient_matches_finite_differences():
+    """Adjoint gradient (f32, GPU) vs central differences of the f64 oracle
+    misfit at 32 random cells, for vp, vs and rho: relative error <= 1e-3."""
+    from seiswave_taichi_amd.adjoint import seismograms
+    m, p, rec = _modeling()
+    vs, vp, rho = layered_model((m.nx, m.nz), seed=4)
+    # observed data: a model with a Gaussian anomaly
+    ii, jj = np.meshgrid(np.arange(m.nx), np.arange(m.nz), indexing="ij")
+    bump = np.exp(-((ii - 30) ** 2 + (jj - 24) ** 2) / (2 * 4.0 ** 2))
+    vp_t, vs_t = (vp * (1 + 0.08 * bump)).astype(np.float32), (vs * (1 - 0.06 * bump)).astype(np.float32)
+    _, dobs = _oracle_misfit(vp_t.astype(np.float64), vs_t.astype(np.float64), rho.astype(np.float64), m, rec, p,
+                             None, m.shots)
+    tv = [torch.from_numpy(a).to("cuda").requires_grad_(True) for a in (vp, vs, rho)]
+    data = seismograms(*tv, m)
+    dob = torch.from_numpy(dobs.astype(np.float32)).to("cuda")
+    loss = 0.5 * torch.sum((data - dob) ** 2)
+    loss.backward()
+    grads = [t.grad.cpu().numpy().astype(np.float64) for t in tv]
+    rng = np.random.default_rng(0)
+    cells = np.stack([rng.integers(12, m.nx - 12, 32), rng.integers(6, m.nz - 12, 32)], 1)
+    base = [vp.astype(np.float64), vs.astype(np.float64), rho.astype(np.float64)]
+    for k, name in enumerate(("vp", "vs", "rho")):
+        fd, ad = [], []
+        for (ci, cj) in cells:
+            h = 1e-4 * base[k][ci, cj]
+            mp = [b.copy() for b in base]
+            mm = [b.copy() for b in base]
+            mp[k][ci, cj] += h
+            mm[k][ci, cj] -= h
+            jp, _ = _oracle_misfit(mp[0], mp[1], mp[2], m, rec, p, dobs, m.shots)
+            jm, _ = _oracle_misfit(mm[0], mm[1], mm[2], m, rec, p, dobs, m.shots)
+            fd.append((jp - jm) / (2 * h))
+            ad.append(grads[k][ci, cj])
+        fd, ad = np.array(fd), np.array(ad)
+        rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+        assert rel <= 1e-3, (name, rel, fd[:4], ad[:4])
