@@ -186,7 +186,8 @@ typedef struct sw_adj2d {
     float *mem;                       /* 10 adjoint memory variables, as sw_ssg2d  */
     float *tmp;                       /* 4 scratch arrays [nz][nx]                 */
     float *g_b1, *g_b4;               /* d misfit / d(dt/rho) at vx, vz cells      */
-    float *g_lamh, *g_muh, *g_muz;    /* d misfit / d(averaged lam, mu)            */
+    float *g_lamh, *g_muh, *g_muz;    /* sxx cells: d/d lam_h and d/d mu_h - 2 d/d lam_h;
+                                         sxz cells: d/d mu_z                        */
 } sw_adj2d;
 
 int sw_ssg2d_adjoint_stress(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream);
@@ -197,6 +198,45 @@ int sw_ssg2d_adjoint_inject(const sw_ssg2d *g, const sw_adj2d *adj, const sw_rec
 /* gvp, gvs, grho [nz][nx] from the accumulated coefficient gradients */
 int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float *vp, const float *vs,
                             float *gvp, float *gvs, float *grho, void *stream);
+
+/* ---- learned thin boundary (no counterpart in the reference: the
+ * north-star replacement of most of a wide CPML by a thin CPML plus a
+ * per-boundary-column MLP) ----
+ * The interior box (cells outside every CPML slab) starts at (ox, oy, oz)
+ * and spans (ex, ey, ez) cells.  Its SW_BAND_DEPTH outermost cells along
+ * each face form the band; a column is one lateral position of one face
+ * (x faces span the full lateral box, y faces exclude the x bands, z faces
+ * the x and y bands).  Per column: 72 features (9 fields x 4 band cells
+ * after the current step, then the same after the previous step) and 36
+ * outputs (corrections of the current band).  Each of the 6 faces has its own
+ * network of sw_mlp_param_count() floats:
+ *   W1[72][64] b1[64] W2[64][64] b2[64] W3[64][36] b3[36] in_scale[72] out_scale[36]
+ *   y = (W3^T tanh(W2^T tanh(W1^T (x/in_scale) + b1) + b2) + b3) * out_scale */
+#define SW_BAND_DEPTH 4
+#define SW_BAND_FEATURES 72
+#define SW_BAND_OUTPUTS 36
+#define SW_MLP_HIDDEN 64
+typedef struct sw_band {
+    int32_t ox, oy, oz, ex, ey, ez;
+} sw_band;
+
+int sw_mlp_param_count(void);
+int sw_band_columns(const sw_band *b);
+/* out[c][36]: current band values of every column */
+int sw_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, void *stream);
+/* fused inference: fields += MLP correction on every band column; prev[c][36]
+ * holds the corrected band of the previous step and is updated in place */
+int sw_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, void *stream);
+/* nsteps of sw_ssg3d_run with sw_band_correct after each stress sweep */
+int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, const sw_band *b,
+                         const float *params, float *prev, int32_t it0, int32_t nsteps, void *stream);
+/* training of one face network on a batch X[B][72] -> targets T[B][36] */
+int sw_mlp_forward(const float *params, const float *X, int32_t B, float *H1, float *H2, float *Y, void *stream);
+/* work: B*(36+128) floats; grad: the trainable part (all but the scales); loss_rows[B] */
+int sw_mlp_loss_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
+                         const float *T, int32_t B, float *grad, float *work, float *loss_rows, void *stream);
+int sw_adam(float *p, const float *g, float *m, float *v, int32_t n, float lr, float beta1, float beta2, float eps,
+            int32_t t, void *stream);
 
 #ifdef __cplusplus
 }

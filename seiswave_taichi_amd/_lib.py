@@ -63,10 +63,15 @@ class SwReceivers(C.Structure):
                 ("idx", C.POINTER(C.c_int32)), ("data", _fp), ("nt", C.c_int32)]
 
 
+class SwBand(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("ox", "oy", "oz", "ex", "ey", "ez")]
+
+
 # Every exported symbol of include/seiswave.h with its argument types.
 _P3, _P2 = C.POINTER(SwSSG3D), C.POINTER(SwSSG2D)
 _PS, _PR = C.POINTER(SwSource), C.POINTER(SwReceivers)
 _PA = C.POINTER(SwAdj2D)
+_PB = C.POINTER(SwBand)
 SIGNATURES = {
     "sw_version": [],
     "sw_last_error": [C.c_char_p, C.c_size_t],
@@ -90,6 +95,15 @@ SIGNATURES = {
     "sw_ssg2d_adjoint_velocity": [_P2, _PA, C.c_void_p],
     "sw_ssg2d_adjoint_inject": [_P2, _PA, _PR, C.c_int32, C.c_void_p],
     "sw_ssg2d_model_gradient": [_P2, _PA, _fp, _fp, _fp, _fp, _fp, C.c_void_p],
+    "sw_mlp_param_count": [],
+    "sw_band_columns": [_PB],
+    "sw_band_gather": [_P3, _PB, _fp, C.c_void_p],
+    "sw_band_correct": [_P3, _PB, _fp, _fp, C.c_void_p],
+    "sw_ssg3d_run_learned": [_P3, _PS, _PR, _PB, _fp, _fp, C.c_int32, C.c_int32, C.c_void_p],
+    "sw_mlp_forward": [_fp, _fp, C.c_int32, _fp, _fp, _fp, C.c_void_p],
+    "sw_mlp_loss_backward": [_fp] * 6 + [C.c_int32, _fp, _fp, _fp, C.c_void_p],
+    "sw_adam": [_fp, _fp, _fp, _fp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
+                C.c_void_p],
 }
 
 _lib = None

@@ -63,8 +63,11 @@ __global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q)
             Dxp = fwd_cpml(Dx, f.mem[6 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
             Dzp = fwd_cpml(Dz, f.mem[9 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
         }
+        // d/d lam_h = (gxx+gzz)(Dx'+Dz'); d/d mu_h = 2(gxx Dx' + gzz Dz').  The vs and
+        // rho gradients need mu_bar - 2 lam_bar, whose two terms nearly cancel for
+        // P-dominated data; accumulate it in its cancellation-free form instead.
         q.g_lamh[p] += (gxx + gzz) * (Dxp + Dzp);
-        q.g_muh[p] += 2.f * (gxx * Dxp + gzz * Dzp);
+        q.g_muh[p] += -2.f * (gxx * Dzp + gzz * Dxp);
         float bx = lpm * gxx + lam_h * gzz;
         float bz = lam_h * gxx + lpm * gzz;
         if (in) {
@@ -249,9 +252,11 @@ __global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const 
     const long SZ = nx;
     const long p = (long)j * SZ + i;
     const float dt = f.dt;
+    // gl = d/d lam; gb = d/d mu - 2 d/d lam (g_muh holds the sxx-cell part of the
+    // latter, g_muz the sxz-cell mu gradient)
     float gl = 0.5f * q.g_lamh[p] + (i >= 1 ? 0.5f * q.g_lamh[p - 1] : 0.f);
-    float gm = 0.5f * q.g_muh[p] + (i >= 1 ? 0.5f * q.g_muh[p - 1] : 0.f);
-    gm += 0.5f * q.g_muz[p] + (j >= 1 ? 0.5f * q.g_muz[p - SZ] : 0.f);
+    float gb = 0.5f * q.g_muh[p] + (i >= 1 ? 0.5f * q.g_muh[p - 1] : 0.f);
+    gb += 0.5f * q.g_muz[p] + (j >= 1 ? 0.5f * q.g_muz[p - SZ] : 0.f);
     const float r = f.rho[p];
     float gr = q.g_b1[p] * (-dt / (r * r));
     // vz cell p' uses rho4 over (p', p'+SZ, p'+1+SZ, p'+1): cell p is in the stencils
@@ -264,10 +269,13 @@ __global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const 
             const float r4 = 0.25f * (f.rho[pp] + f.rho[pp + SZ] + f.rho[pp + 1 + SZ] + f.rho[pp + 1]);
             gr += 0.25f * q.g_b4[pp] * (-dt / (r4 * r4));
         }
+    // lam = rho (vp^2 - 2 vs^2), mu = rho vs^2:
+    //   d/dvp = 2 rho vp lam_bar;  d/dvs = 2 rho vs (mu_bar - 2 lam_bar);
+    //   d/drho (moduli part) = lam_bar vp^2 + vs^2 (mu_bar - 2 lam_bar)
     const float a = vp[p], s = vs[p];
     gvp[p] = gl * 2.f * r * a;
-    gvs[p] = gl * (-4.f * r * s) + gm * 2.f * r * s;
-    grho[p] = gr + gl * (a * a - 2.f * s * s) + gm * s * s;
+    gvs[p] = gb * 2.f * r * s;
+    grho[p] = gr + gl * (a * a) + gb * (s * s);
 }
 
 inline dim3 grid2(const sw_ssg2d &g, dim3 blk)

@@ -25,6 +25,15 @@ int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st);
 int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st);
 int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
+int learned_band_columns(const sw_band *b);
+int launch_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, hipStream_t st);
+int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, hipStream_t st);
+int launch_mlp_forward(const float *params, const float *X, int B, float *H1, float *H2, float *Y, hipStream_t st);
+int launch_mlp_loss_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
+                             const float *T, int B, float *grad, float *work, float *loss_rows, hipStream_t st);
+int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, int t,
+                hipStream_t st);
+int mlp_param_count();
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
 int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st);
 int launch_adj2d_model_gradient(const sw_ssg2d *f, const sw_adj2d *q, const float *vp, const float *vs, float *gvp,
@@ -467,6 +476,90 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
     if (rc) return rc;
     SW_REQUIRE(vp && vs && gvp && gvs && grho, "model gradient arrays missing");
     return launch_adj2d_model_gradient(g, adj, vp, vs, gvp, gvs, grho, as_stream(stream));
+}
+
+static int check_band(const sw_ssg3d *g, const sw_band *b)
+{
+    SW_REQUIRE(b != nullptr, "null band");
+    SW_REQUIRE(b->ex >= 2 * SW_BAND_DEPTH + 1 && b->ey >= 2 * SW_BAND_DEPTH + 1 && b->ez >= 2 * SW_BAND_DEPTH + 1,
+               "interior box %dx%dx%d too small for a band of depth %d", b->ex, b->ey, b->ez, SW_BAND_DEPTH);
+    SW_REQUIRE(b->ox >= 0 && b->oy >= 0 && b->oz >= 0 && b->ox + b->ex <= g->nx && b->oy + b->ey <= g->ny &&
+                   b->oz + b->ez <= g->nz,
+               "interior box outside the grid");
+    return SW_OK;
+}
+
+int sw_mlp_param_count(void) { return mlp_param_count(); }
+
+int sw_band_columns(const sw_band *b)
+{
+    SW_REQUIRE(b != nullptr, "null band");
+    return learned_band_columns(b);
+}
+
+int sw_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, void *stream)
+{
+    int rc = check_ssg3d(g, g && g->cls == SW_CLASS_SPML);
+    if (rc) return rc;
+    if ((rc = check_band(g, b))) return rc;
+    SW_REQUIRE(out != nullptr, "null output");
+    return launch_band_gather(g, b, out, as_stream(stream));
+}
+
+int sw_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, void *stream)
+{
+    int rc = check_ssg3d(g, g && g->cls == SW_CLASS_SPML);
+    if (rc) return rc;
+    if ((rc = check_band(g, b))) return rc;
+    SW_REQUIRE(params && prev, "null network or band state");
+    return launch_band_correct(g, b, params, prev, as_stream(stream));
+}
+
+int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, const sw_band *b,
+                         const float *params, float *prev, int32_t it0, int32_t nsteps, void *stream)
+{
+    const bool spml = g && g->cls == SW_CLASS_SPML;
+    int rc = check_ssg3d(g, spml);
+    if (rc) return rc;
+    if ((rc = check_band(g, b))) return rc;
+    SW_REQUIRE(params && prev, "null network or band state");
+    SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");
+    if (nsteps == 0) return SW_OK;
+    if ((rc = check_source3d(g, src, it0)) || (rc = check_source3d(g, src, it0 + nsteps - 1))) return rc;
+    if (rec && ((rc = check_receivers(rec, it0)) || (rc = check_receivers(rec, it0 + nsteps - 1)))) return rc;
+    hipStream_t st = as_stream(stream);
+    const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
+    const FieldPtrs fp = fields3d(g);
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        if (pre) hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = spml ? launch_ssg3d_velocity_spml(g, st) : launch_ssg3d_velocity(g, st))) return rc;
+        if (!pre && src->kind != SW_SRC_NONE) hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = spml ? launch_ssg3d_stress_spml(g, st) : launch_ssg3d_stress(g, st))) return rc;
+        if ((rc = launch_band_correct(g, b, params, prev, st))) return rc;
+        if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
+    }
+    return launch_status("ssg3d learned run");
+}
+
+int sw_mlp_forward(const float *params, const float *X, int32_t B, float *H1, float *H2, float *Y, void *stream)
+{
+    SW_REQUIRE(params && X && H1 && H2 && Y && B > 0, "bad mlp forward arguments");
+    return launch_mlp_forward(params, X, B, H1, H2, Y, as_stream(stream));
+}
+
+int sw_mlp_loss_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
+                         const float *T, int32_t B, float *grad, float *work, float *loss_rows, void *stream)
+{
+    SW_REQUIRE(params && X && H1 && H2 && Y && T && grad && work && loss_rows && B > 0,
+               "bad mlp backward arguments");
+    return launch_mlp_loss_backward(params, X, H1, H2, Y, T, B, grad, work, loss_rows, as_stream(stream));
+}
+
+int sw_adam(float *p, const float *g, float *m, float *v, int32_t n, float lr, float beta1, float beta2, float eps,
+            int32_t t, void *stream)
+{
+    SW_REQUIRE(p && g && m && v && n > 0 && t >= 1, "bad adam arguments");
+    return launch_adam(p, g, m, v, n, lr, beta1, beta2, eps, t, as_stream(stream));
 }
 
 }  // extern "C"

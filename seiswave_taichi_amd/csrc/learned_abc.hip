@@ -1,0 +1,331 @@
+// Learned thin absorbing boundary: a per-boundary-column MLP that corrects
+// the fields next to a thin CPML (no counterpart in the reference; the
+// north-star extension that replaces most of a wide CPML).
+//
+// Geometry.  The "interior" is the box of cells outside every CPML slab.  A
+// band is the D = 4 outermost interior cells along one face normal; each band
+// column (one lateral position of one face) is one MLP sample.  Faces are
+// numbered x-lo, x-hi, y-lo, y-hi, z-lo, z-hi; x faces span the full lateral
+// y-z extent, y faces exclude the x bands, z faces exclude the x and y bands,
+// so every band cell belongs to exactly one column.
+//
+// Features of a column: the 9 fields (vx vy vz sxx syy szz sxy sxz syz) at
+// the D band cells after the current step, then the same 36 values after the
+// previous step (as corrected), i.e. 72 = 9 x 4 x 2; band cell d = 0 touches
+// the CPML.  Output: 36 corrections (same order as the first half).
+//
+// Network of one face (params, row-major):
+//   W1[72][64] b1[64] W2[64][64] b2[64] W3[64][36] b3[36] in_scale[72] out_scale[36]
+//   y = (W3^T tanh(W2^T tanh(W1^T (x / in_scale) + b1) + b2) + b3) * out_scale
+#include "sw_internal.h"
+
+namespace sw {
+
+namespace {
+
+constexpr int kD = 4, kF = 9, kIn = 2 * kD * kF, kOut = kD * kF, kH = 64;
+constexpr int kParams = kIn * kH + kH + kH * kH + kH + kH * kOut + kOut + kIn + kOut;
+
+struct BandGeo {
+    int ox, oy, oz, ex, ey, ez;
+    int ncol[6];
+    int col0[7];
+};
+
+BandGeo band_geo(const sw_band &b)
+{
+    BandGeo q;
+    q.ox = b.ox; q.oy = b.oy; q.oz = b.oz; q.ex = b.ex; q.ey = b.ey; q.ez = b.ez;
+    const int ex2 = b.ex - 2 * kD, ey2 = b.ey - 2 * kD;
+    q.ncol[0] = q.ncol[1] = b.ey * b.ez;
+    q.ncol[2] = q.ncol[3] = ex2 * b.ez;
+    q.ncol[4] = q.ncol[5] = ex2 * ey2;
+    q.col0[0] = 0;
+    for (int f = 0; f < 6; ++f) q.col0[f + 1] = q.col0[f] + q.ncol[f];
+    return q;
+}
+
+// Cell (i, j, k) of band depth d of column `a, b` of face f.
+__device__ __forceinline__ void band_cell(const BandGeo &q, int f, int a, int b, int d, int &i, int &j, int &k)
+{
+    switch (f) {
+    case 0: i = q.ox + d; j = q.oy + a; k = q.oz + b; break;
+    case 1: i = q.ox + q.ex - 1 - d; j = q.oy + a; k = q.oz + b; break;
+    case 2: i = q.ox + kD + a; j = q.oy + d; k = q.oz + b; break;
+    case 3: i = q.ox + kD + a; j = q.oy + q.ey - 1 - d; k = q.oz + b; break;
+    case 4: i = q.ox + kD + a; j = q.oy + kD + b; k = q.oz + d; break;
+    default: i = q.ox + kD + a; j = q.oy + kD + b; k = q.oz + q.ez - 1 - d; break;
+    }
+}
+
+__device__ __forceinline__ int face_of(const BandGeo &q, int c)
+{
+    int f = 0;
+#pragma unroll
+    for (int t = 1; t < 6; ++t) f += (c >= q.col0[t]);
+    return f;
+}
+
+__device__ __forceinline__ int lateral_a_extent(const BandGeo &q, int f)
+{
+    return f < 2 ? q.ey : q.ex - 2 * kD;
+}
+
+struct Fields9 {
+    float *f[kF];
+};
+
+Fields9 fields9(const sw_ssg3d &g)
+{
+    return Fields9{{g.vx, g.vy, g.vz, g.sxx, g.syy, g.szz, g.sxy, g.sxz, g.syz}};
+}
+
+// Current band values of every column: out[c][d*9 + field].
+__global__ void k_band_gather(Fields9 F, int nx, int ny, BandGeo q, float *out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= q.col0[6]) return;
+    const int f = face_of(q, c);
+    const int r = c - q.col0[f];
+    const int na = lateral_a_extent(q, f);
+    const int a = r % na, b = r / na;
+    for (int d = 0; d < kD; ++d) {
+        int i, j, k;
+        band_cell(q, f, a, b, d, i, j, k);
+        const long p = ((long)k * ny + j) * nx + i;
+#pragma unroll
+        for (int t = 0; t < kF; ++t) out[(long)c * kOut + d * kF + t] = F.f[t][p];
+    }
+}
+
+// Fused inference: gather, MLP, correct the fields, keep the corrected band.
+// Blocks never straddle faces: block b serves face blk_face[b].
+constexpr int kCorrectThreads = 256;
+
+struct FaceBlocks {
+    int blk0[7];  // first block of each face
+};
+
+__global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, FaceBlocks fb,
+                                                                  const float *params, float *prev)
+{
+    __shared__ float w[kParams];
+    int f = 0;
+#pragma unroll
+    for (int t = 1; t < 6; ++t) f += ((int)blockIdx.x >= fb.blk0[t]);
+    const float *P = params + (long)f * kParams;
+    for (int t = threadIdx.x; t < kParams; t += blockDim.x) w[t] = P[t];
+    __syncthreads();
+    const int r = ((int)blockIdx.x - fb.blk0[f]) * blockDim.x + threadIdx.x;
+    if (r >= q.ncol[f]) return;
+    const int c = q.col0[f] + r;
+    const int na = lateral_a_extent(q, f);
+    const int a = r % na, b = r / na;
+    const float *W1 = w, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH, *B3 = W3 + kH * kOut;
+    const float *IS = B3 + kOut, *OS = IS + kIn;
+    float x[kIn];
+    long pc[kD];
+#pragma unroll
+    for (int d = 0; d < kD; ++d) {
+        int i, j, k;
+        band_cell(q, f, a, b, d, i, j, k);
+        pc[d] = ((long)k * ny + j) * nx + i;
+#pragma unroll
+        for (int t = 0; t < kF; ++t) x[d * kF + t] = F.f[t][pc[d]];
+    }
+#pragma unroll
+    for (int t = 0; t < kOut; ++t) x[kOut + t] = prev[(long)c * kOut + t];
+    float h1[kH];
+#pragma unroll
+    for (int o = 0; o < kH; ++o) h1[o] = B1[o];
+    for (int in = 0; in < kIn; ++in) {
+        const float xi = x[in] / IS[in];
+#pragma unroll
+        for (int o = 0; o < kH; ++o) h1[o] += xi * W1[in * kH + o];
+    }
+#pragma unroll
+    for (int o = 0; o < kH; ++o) h1[o] = tanhf(h1[o]);
+    float h2[kH];
+#pragma unroll
+    for (int o = 0; o < kH; ++o) h2[o] = B2[o];
+    for (int in = 0; in < kH; ++in) {
+        const float hi = h1[in];
+#pragma unroll
+        for (int o = 0; o < kH; ++o) h2[o] += hi * W2[in * kH + o];
+    }
+#pragma unroll
+    for (int o = 0; o < kH; ++o) h2[o] = tanhf(h2[o]);
+    float y[kOut];
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) y[o] = B3[o];
+    for (int in = 0; in < kH; ++in) {
+        const float hi = h2[in];
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) y[o] += hi * W3[in * kOut + o];
+    }
+#pragma unroll
+    for (int d = 0; d < kD; ++d)
+#pragma unroll
+        for (int t = 0; t < kF; ++t) {
+            const int o = d * kF + t;
+            const float v = x[o] + y[o] * OS[o];
+            F.f[t][pc[d]] = v;
+            prev[(long)c * kOut + o] = v;
+        }
+}
+
+// ------------------------------------------------------------ training
+// Dense layer Z = A W + b (A [B][nin], W [nin][nout]); act 1 applies tanh.
+// The first layer divides its input by in_scale (when given).
+__global__ void k_dense(const float *A, const float *W, const float *bias, const float *in_scale, float *Z, int B,
+                        int nin, int nout, int act)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)B * nout) return;
+    const int o = (int)(t % nout);
+    const long row = t / nout;
+    float acc = bias[o];
+    for (int i = 0; i < nin; ++i) {
+        const float a = in_scale ? A[row * nin + i] / in_scale[i] : A[row * nin + i];
+        acc += a * W[(long)i * nout + o];
+    }
+    Z[t] = act ? tanhf(acc) : acc;
+}
+
+// dZ of a tanh layer from the gradient of its output: dZ = (dH W_next^T) * (1 - H^2),
+// with dH given as the next layer's dZ [B][nnext] and W_next [n][nnext].
+__global__ void k_dense_back(const float *dnext, const float *Wnext, const float *H, float *dZ, int B, int n,
+                             int nnext)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)B * n) return;
+    const int i = (int)(t % n);
+    const long row = t / n;
+    float acc = 0.f;
+    for (int o = 0; o < nnext; ++o) acc += dnext[row * nnext + o] * Wnext[(long)i * nnext + o];
+    const float h = H[t];
+    dZ[t] = acc * (1.f - h * h);
+}
+
+// dW[i][o] = sum_b A[b][i] dZ[b][o] (A scaled by 1/in_scale when given); db[o] = sum_b dZ[b][o].
+__global__ void k_dense_wgrad(const float *A, const float *in_scale, const float *dZ, float *dW, float *db, int B,
+                              int nin, int nout)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nin * nout) {
+        const int i = t / nout, o = t % nout;
+        const float sc = in_scale ? 1.f / in_scale[i] : 1.f;
+        float acc = 0.f;
+        for (int b = 0; b < B; ++b) acc += A[(long)b * nin + i] * dZ[(long)b * nout + o];
+        dW[t] = acc * sc;
+    } else if (t < nin * nout + nout) {
+        const int o = t - nin * nout;
+        float acc = 0.f;
+        for (int b = 0; b < B; ++b) acc += dZ[(long)b * nout + o];
+        db[o] = acc;
+    }
+}
+
+// Mean squared error in normalised output units: L = mean((Y - T/out_scale)^2);
+// dY = 2 (Y - T/out_scale) / (B*nout); loss_parts[b] = row sum of squares.
+__global__ void k_mse(const float *Y, const float *T, const float *out_scale, float *dY, float *loss_rows, int B,
+                      int nout)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float acc = 0.f;
+    const float inv = 2.f / ((float)B * (float)nout);
+    for (int o = 0; o < nout; ++o) {
+        const float e = Y[(long)b * nout + o] - T[(long)b * nout + o] / out_scale[o];
+        dY[(long)b * nout + o] = e * inv;
+        acc += e * e;
+    }
+    loss_rows[b] = acc;
+}
+
+__global__ void k_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gt = g[t];
+    const float mt = b1 * m[t] + (1.f - b1) * gt;
+    const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+    m[t] = mt;
+    v[t] = vt;
+    p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+}
+
+inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+int learned_band_columns(const sw_band *b)
+{
+    return band_geo(*b).col0[6];
+}
+
+int launch_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, hipStream_t st)
+{
+    const BandGeo q = band_geo(*b);
+    hipLaunchKernelGGL(k_band_gather, dim3(nblk(q.col0[6], 256)), dim3(256), 0, st, fields9(*g), g->nx, g->ny, q,
+                       out);
+    return launch_status("band gather");
+}
+
+int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, hipStream_t st)
+{
+    const BandGeo q = band_geo(*b);
+    FaceBlocks fb;
+    fb.blk0[0] = 0;
+    for (int f = 0; f < 6; ++f) fb.blk0[f + 1] = fb.blk0[f] + (int)nblk(q.ncol[f], kCorrectThreads);
+    if (fb.blk0[6] == 0) return SW_OK;
+    hipLaunchKernelGGL(k_band_correct, dim3(fb.blk0[6]), dim3(kCorrectThreads), 0, st, fields9(*g), g->nx, g->ny, q,
+                       fb, params, prev);
+    return launch_status("band correction");
+}
+
+int launch_mlp_forward(const float *params, const float *X, int B, float *H1, float *H2, float *Y, hipStream_t st)
+{
+    const float *W1 = params, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH;
+    const float *B3 = W3 + kH * kOut, *IS = B3 + kOut;
+    hipLaunchKernelGGL(k_dense, dim3(nblk((long)B * kH, 256)), dim3(256), 0, st, X, W1, B1, IS, H1, B, kIn, kH, 1);
+    hipLaunchKernelGGL(k_dense, dim3(nblk((long)B * kH, 256)), dim3(256), 0, st, H1, W2, B2, (const float *)nullptr,
+                       H2, B, kH, kH, 1);
+    hipLaunchKernelGGL(k_dense, dim3(nblk((long)B * kOut, 256)), dim3(256), 0, st, H2, W3, B3,
+                       (const float *)nullptr, Y, B, kH, kOut, 0);
+    return launch_status("mlp forward");
+}
+
+int launch_mlp_loss_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
+                             const float *T, int B, float *grad, float *work, float *loss_rows, hipStream_t st)
+{
+    const float *W1 = params, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH;
+    const float *B3 = W3 + kH * kOut, *IS = B3 + kOut, *OS = IS + kIn;
+    (void)B1; (void)B2;
+    float *gW1 = grad, *gB1 = gW1 + kIn * kH, *gW2 = gB1 + kH, *gB2 = gW2 + kH * kH, *gW3 = gB2 + kH;
+    float *gB3 = gW3 + kH * kOut;
+    float *dY = work, *dZ2 = dY + (long)B * kOut, *dZ1 = dZ2 + (long)B * kH;
+    hipLaunchKernelGGL(k_mse, dim3(nblk(B, 256)), dim3(256), 0, st, Y, T, OS, dY, loss_rows, B, kOut);
+    hipLaunchKernelGGL(k_dense_wgrad, dim3(nblk(kH * kOut + kOut, 256)), dim3(256), 0, st, H2,
+                       (const float *)nullptr, dY, gW3, gB3, B, kH, kOut);
+    hipLaunchKernelGGL(k_dense_back, dim3(nblk((long)B * kH, 256)), dim3(256), 0, st, dY, W3, H2, dZ2, B, kH, kOut);
+    hipLaunchKernelGGL(k_dense_wgrad, dim3(nblk(kH * kH + kH, 256)), dim3(256), 0, st, H1, (const float *)nullptr,
+                       dZ2, gW2, gB2, B, kH, kH);
+    hipLaunchKernelGGL(k_dense_back, dim3(nblk((long)B * kH, 256)), dim3(256), 0, st, dZ2, W2, H1, dZ1, B, kH, kH);
+    hipLaunchKernelGGL(k_dense_wgrad, dim3(nblk(kIn * kH + kH, 256)), dim3(256), 0, st, X, IS, dZ1, gW1, gB1, B, kIn,
+                       kH);
+    return launch_status("mlp backward");
+}
+
+int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, int t,
+                hipStream_t st)
+{
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(n, 256)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    return launch_status("adam");
+}
+
+int mlp_param_count() { return kParams; }
+
+}  // namespace sw

@@ -5,15 +5,27 @@
 //
 // Execution: each thread owns one (i, j) column of an x-y tile (x fastest,
 // 64 lanes = one wavefront row) and marches along z over a chunk of planes.
-// The z-differentiated fields live in per-thread register windows of 2N
-// values that slide with k, so each z plane of those fields is read from
-// memory once per column; x and y taps are plain loads that hit the L1/L2
-// lines the neighbouring lanes and waves of the tile already fetched.  Blocks
-// are renumbered so that the tiles of one x-y neighbourhood run on the same
-// XCD (MI355X dispatches consecutive workgroups round-robin over 8 XCDs, each
-// with its own L2).  The z-slab of the CPML is a wave-uniform branch; the x
-// and y slabs are per-lane.
+//  * The z-differentiated fields live in per-thread register windows of 2N
+//    values that slide with k, so each of their planes is read once.
+//  * Every value a plane needs (x and y stencil taps, material, the
+//    read-modify-write fields, CPML memory variables and coefficients) is
+//    gathered into a register "plane" before any arithmetic, and the next
+//    plane is fetched while the current one is computed: each wave keeps
+//    ~35 independent loads in flight instead of waiting on one branch at a
+//    time.  All addresses of an active column are in bounds, so the loads
+//    need no per-component predicates.
+//  * Blocks are renumbered so that the tiles of one neighbourhood run on the
+//    same XCD (MI355X dispatches consecutive workgroups round-robin over 8
+//    XCDs, each with its own L2) and halo lines are shared within one L2.
+//  * The z slab of the CPML is a wave-uniform condition; x and y slabs are
+//    per lane.
+// Arithmetic is term for term the reference's (build with -ffp-contract=off).
 #include "sw_internal.h"
+
+// SW_PREFETCH=1 double-buffers the register plane (fetch k+1 while computing k).
+#ifndef SW_PREFETCH
+#define SW_PREFETCH 1
+#endif
 
 namespace sw {
 
@@ -21,8 +33,6 @@ namespace {
 
 constexpr int kXCD = 8;  // XCDs per MI355X; any value keeps the renumbering a bijection
 
-// Physical workgroup b runs on XCD b % kXCD.  Give each XCD a contiguous range
-// of logical tiles so halo lines are shared within one L2.
 __device__ __forceinline__ int xcd_logical_block(int b, int nb)
 {
     const int q = nb / kXCD, r = nb % kXCD;
@@ -54,9 +64,65 @@ __device__ __forceinline__ Cell tile_cell(const sw_ssg3d &g, const Geo geo, int 
     return c;
 }
 
-// z taps from a window w[t] = f[k - N + t] (t = 0..2N-1): backward difference
+// Fields are addressed by 32-bit byte offsets from their base pointers: one
+// offset register per column serves every field, and the wave-uniform tap
+// deltas fold into the load instructions.  The host keeps every array below
+// 4 GiB.
+__device__ __forceinline__ float ld(const float *__restrict__ base, uint32_t off)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+
+__device__ __forceinline__ void st(float *__restrict__ base, uint32_t off, float v)
+{
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// Load at byte offset v + d, d a wave-uniform non-negative delta.
+__device__ __forceinline__ float ldu(const float *__restrict__ base, uint32_t v, int d)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + v + (uint32_t)d);
+}
+
+// Base pointers of all the arrays a 3-D sweep touches.
+struct Res {
+    float *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz, *mx, *my, *mz;
+    const float *rho, *lam, *mu;
+};
+
+__device__ __forceinline__ Res make_res(const sw_ssg3d &g)
+{
+    Res r;
+    r.vx = g.vx; r.vy = g.vy; r.vz = g.vz;
+    r.sxx = g.sxx; r.syy = g.syy; r.szz = g.szz;
+    r.sxy = g.sxy; r.sxz = g.sxz; r.syz = g.syz;
+    r.rho = g.rho; r.lam = g.lam; r.mu = g.mu;
+    r.mx = g.mem_x; r.my = g.mem_y; r.mz = g.mem_z;
+    return r;
+}
+
+// Tap arrays: a "bm" array holds f[p + (t-N)*st] (offsets -N..N-1), an "fm"
+// array f[p + (t-N+1)*st] (offsets -N+1..N), t = 0..2N-1.  All loads of a
+// plane are addressed from vb = byte offset of (i-N, j-N, k); c0 is the
+// uniform delta of (i, j, k) from there and st the tap stride, in bytes.
 template <int N>
-__device__ __forceinline__ float win_bm(const float *w, const float *c)
+__device__ __forceinline__ void taps_bm(float *w, const float *__restrict__ f, uint32_t vb, int c0, int st)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t) w[t] = ldu(f, vb, c0 + (t - N) * st);
+}
+
+template <int N>
+__device__ __forceinline__ void taps_fm(float *w, const float *__restrict__ f, uint32_t vb, int c0, int st)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t) w[t] = ldu(f, vb, c0 + (t - N + 1) * st);
+}
+
+// Staggered differences from tap arrays, accumulated exactly as
+// src/Differential3D.py:7-44 does (0, then c[id]*(difference) in id order).
+template <int N>
+__device__ __forceinline__ float diff_bm(const float *w, const float *c)
 {
     float v = 0.f;
 #pragma unroll
@@ -64,9 +130,8 @@ __device__ __forceinline__ float win_bm(const float *w, const float *c)
     return v;
 }
 
-// forward difference from a window w[t] = f[k - N + 1 + t]
 template <int N>
-__device__ __forceinline__ float win_fm(const float *w, const float *c)
+__device__ __forceinline__ float diff_fm(const float *w, const float *c)
 {
     float v = 0.f;
 #pragma unroll
@@ -81,30 +146,113 @@ __device__ __forceinline__ void win_shift(float *w)
     for (int t = 0; t < 2 * N - 1; ++t) w[t] = w[t + 1];
 }
 
-struct SlabIdx {
-    int xs, ys;        // slab index along x / y (-1 outside)
-    long rx, ry;       // (k*ny + j)*wx + xs at k = 0, and (k*wy + ys)*nx + i at k = 0
-    long sx, sy, sz;   // per-k stride of the x / y slab arrays, z-slab row pitch
-    long nxs, nys, nzs;
+template <int M>
+__device__ __forceinline__ void copy_n(float *d, const float *s)
+{
+#pragma unroll
+    for (int t = 0; t < M; ++t) d[t] = s[t];
+}
+
+// Byte offsets of one column into the CPML slab arrays.
+struct Slab {
+    int xs, ys;
+    uint32_t rx, ry;      // x / y slab offset at k = 0
+    uint32_t sx, sy, sz;  // per-k strides of the x / y slab arrays; plane size of the z slab
+    uint32_t nxs, nys, nzs;
 };
 
-__device__ __forceinline__ SlabIdx slab_setup(const sw_ssg3d &g, int i, int j)
+__device__ __forceinline__ Slab slab_setup(const sw_ssg3d &g, int i, int j)
 {
-    SlabIdx s;
+    Slab s;
     const int wx = g.px.lo + (g.nx - g.px.hi);
     const int wy = g.py.lo + (g.ny - g.py.hi);
     const int wz = g.pz.lo + (g.nz - g.pz.hi);
     s.xs = slab_index(i, g.px.lo, g.px.hi);
     s.ys = slab_index(j, g.py.lo, g.py.hi);
-    s.rx = (long)j * wx + s.xs;
-    s.ry = (long)s.ys * g.nx + i;
-    s.sx = (long)g.ny * wx;
-    s.sy = (long)wy * g.nx;
-    s.sz = (long)g.ny * g.nx;
-    s.nxs = (long)g.nz * g.ny * wx;
-    s.nys = (long)g.nz * wy * g.nx;
-    s.nzs = (long)wz * g.ny * g.nx;
+    s.rx = 4u * (uint32_t)(j * wx + s.xs);
+    s.ry = 4u * (uint32_t)(s.ys * g.nx + i);
+    s.sx = 4u * (uint32_t)(g.ny * wx);
+    s.sy = 4u * (uint32_t)(wy * g.nx);
+    s.sz = 4u * (uint32_t)(g.ny * g.nx);
+    s.nxs = s.sx * (uint32_t)g.nz;
+    s.nys = s.sy * (uint32_t)g.nz;
+    s.nzs = s.sz * (uint32_t)wz;
     return s;
+}
+
+// CPML coefficients of one point: integer (a, b, k) and half (ah, bh, kh).
+struct Coef {
+    float a, b, k, ah, bh, kh;
+};
+
+__device__ __forceinline__ Coef coef_of(const sw_axis &ax, int i, bool in_slab)
+{
+    Coef c{0.f, 1.f, 1.f, 0.f, 1.f, 1.f};
+    if (in_slab) {
+        c.a = ax.a[i]; c.b = ax.b[i]; c.k = ax.k[i];
+        c.ah = ax.a_h[i]; c.bh = ax.b_h[i]; c.kh = ax.k_h[i];
+    }
+    return c;
+}
+
+// m = b*m + a*D; D = D/k + m  (src/Elastic3D_SSG_CPML.py:172-177)
+__device__ __forceinline__ float cpml_step(float D, float &m, float a, float b, float k)
+{
+    m = b * m + a * D;
+    return D / k + m;
+}
+
+// ------------------------------------------------------------ velocity sweep
+template <int N>
+struct VPlane {
+    float sxx[2 * N];   // fm taps along x (Dx3fm, vx)
+    float sxy_x[2 * N]; // bm taps along x (Dx3bm, vy)
+    float sxy_y[2 * N]; // bm taps along y (Dy3bm, vx)
+    float syy[2 * N];   // fm taps along y (Dy3fm, vy)
+    float sxz_x[2 * N]; // bm taps along x (Dx3bm, vz)
+    float syz_y[2 * N]; // bm taps along y (Dy3bm, vz)
+    float rho[4];       // p, p+SY, p+1+SY, p+1
+    float v[3];
+    float wxz, wyz, wzz; // next window entries: sxz, syz at k+N-1, szz at k+N
+    float mx[3], my[3], mz[3];
+    Coef cz;
+    int zs;
+};
+
+// p: byte offset of (i, j, k); pc: of (i, j, 0); SY, SZ: byte strides.
+template <int N>
+__device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
+                                            const Slab &sl)
+{
+    // vb: byte offset of (i-N, j-N, k); every load below is vb + a uniform delta
+    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
+    const int c0 = N * 4 + N * SY;
+    taps_fm<N>(P.sxx, R.sxx, vb, c0, 4);
+    taps_bm<N>(P.sxy_x, R.sxy, vb, c0, 4);
+    taps_bm<N>(P.sxy_y, R.sxy, vb, c0, SY);
+    taps_fm<N>(P.syy, R.syy, vb, c0, SY);
+    taps_bm<N>(P.sxz_x, R.sxz, vb, c0, 4);
+    taps_bm<N>(P.syz_y, R.syz, vb, c0, SY);
+    P.rho[0] = ldu(R.rho, vb, c0);
+    P.rho[1] = ldu(R.rho, vb, c0 + SY);
+    P.rho[2] = ldu(R.rho, vb, c0 + 4 + SY);
+    P.rho[3] = ldu(R.rho, vb, c0 + 4);
+    P.v[0] = ldu(R.vx, vb, c0);
+    P.v[1] = ldu(R.vy, vb, c0);
+    P.v[2] = ldu(R.vz, vb, c0);
+    P.wxz = ldu(R.sxz, vb, c0 + (N - 1) * (int)SZ);
+    P.wyz = ldu(R.syz, vb, c0 + (N - 1) * (int)SZ);
+    P.wzz = ldu(R.szz, vb, c0 + N * (int)SZ);
+    P.zs = slab_index(k, g.pz.lo, g.pz.hi);
+    P.cz = coef_of(g.pz, k, P.zs >= 0);
+    const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+    const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        P.mx[m] = sl.xs >= 0 ? ldu(R.mx, ox, (int)(m * sl.nxs)) : 0.f;
+        P.my[m] = sl.ys >= 0 ? ldu(R.my, oy, (int)(m * sl.nys)) : 0.f;
+        P.mz[m] = P.zs >= 0 ? ldu(R.mz, oz, (int)(m * sl.nzs)) : 0.f;
+    }
 }
 
 template <int N, int CLS, int TX, int TY>
@@ -114,7 +262,8 @@ __global__ void __launch_bounds__(TX *TY) k_velocity(sw_ssg3d g, Geo geo)
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (i < N || i >= nx - N || j < N || j >= ny - N) return;
-    const long SY = nx, SZ = (long)nx * ny;
+    const int SY = 4 * nx;
+    const uint32_t SZ = 4u * (uint32_t)(nx * ny);
     const float dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
     float c[N];
 #pragma unroll
@@ -122,76 +271,163 @@ __global__ void __launch_bounds__(TX *TY) k_velocity(sw_ssg3d g, Geo geo)
     const bool vx_col = i < nx - N - 1 && j >= N + 1;
     const bool vy_col = j < ny - N - 1 && i >= N + 1;
     const bool vz_col = i >= N + 1 && j >= N + 1;
-    const SlabIdx sl = slab_setup(g, i, j);
-    // per-column CPML coefficients along x and y
-    float ax_h = 0.f, bx_h = 1.f, kx_h = 1.f, ax = 0.f, bx = 1.f, kx = 1.f;
-    float ay_h = 0.f, by_h = 1.f, ky_h = 1.f, ay = 0.f, by = 1.f, ky = 1.f;
-    if (sl.xs >= 0) {
-        ax_h = g.px.a_h[i]; bx_h = g.px.b_h[i]; kx_h = g.px.k_h[i];
-        ax = g.px.a[i]; bx = g.px.b[i]; kx = g.px.k[i];
-    }
-    if (sl.ys >= 0) {
-        ay_h = g.py.a_h[j]; by_h = g.py.b_h[j]; ky_h = g.py.k_h[j];
-        ay = g.py.a[j]; by = g.py.b[j]; ky = g.py.k[j];
-    }
-    const long pc = (long)j * SY + i;  // column base
+    const Slab sl = slab_setup(g, i, j);
+    const Res R = make_res(g);
+    const Coef cx = coef_of(g.px, i, sl.xs >= 0);
+    const Coef cy = coef_of(g.py, j, sl.ys >= 0);
+    const uint32_t pc = 4u * (uint32_t)(j * nx + i);
     // windows: wxz[t] = sxz[k-N+t], wyz[t] = syz[k-N+t], wzz[t] = szz[k-N+1+t]
     float wxz[2 * N], wyz[2 * N], wzz[2 * N];
     const int k0 = cl.k0;
 #pragma unroll
     for (int t = 0; t < 2 * N - 1; ++t) {
-        wxz[t] = g.sxz[pc + (long)(k0 - N + t) * SZ];
-        wyz[t] = g.syz[pc + (long)(k0 - N + t) * SZ];
-        wzz[t] = g.szz[pc + (long)(k0 - N + 1 + t) * SZ];
+        wxz[t] = ld(R.sxz, pc + (uint32_t)(k0 - N + t) * SZ);
+        wyz[t] = ld(R.syz, pc + (uint32_t)(k0 - N + t) * SZ);
+        wzz[t] = ld(R.szz, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
     }
+    VPlane<N> P;
+#if SW_PREFETCH
+    load_vplane<N>(P, g, R, pc, k0, SY, SZ, sl);
+#endif
     for (int k = k0; k < cl.k1; ++k) {
-        const long p = pc + (long)k * SZ;
-        wxz[2 * N - 1] = g.sxz[p + (long)(N - 1) * SZ];
-        wyz[2 * N - 1] = g.syz[p + (long)(N - 1) * SZ];
-        wzz[2 * N - 1] = g.szz[p + (long)N * SZ];
-        const int zs = slab_index(k, g.pz.lo, g.pz.hi);  // wave-uniform
+#if SW_PREFETCH
+        VPlane<N> Q;
+        if (k + 1 < cl.k1) load_vplane<N>(Q, g, R, pc, k + 1, SY, SZ, sl);
+#else
+        load_vplane<N>(P, g, R, pc, k, SY, SZ, sl);
+#endif
+        const uint32_t p = pc + (uint32_t)k * SZ;
+        const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+        const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+        wxz[2 * N - 1] = P.wxz;
+        wyz[2 * N - 1] = P.wyz;
+        wzz[2 * N - 1] = P.wzz;
         // vx at (i+1/2, j, k): src/Elastic3D_SSG_CPML.py:164-178
         if (vx_col && k >= N + 1) {
-            float dsxx = d_fm<N>(g.sxx, p, 1, c) / dx;
-            float dsxy = d_bm<N>(g.sxy, p, SY, c) / dy;
-            float dsxz = win_bm<N>(wxz, c) / dz;
-            if (sl.xs >= 0) dsxx = cpml_apply(dsxx, g.mem_x + 0 * sl.nxs + sl.rx + k * sl.sx, ax_h, bx_h, kx_h);
-            if (sl.ys >= 0) dsxy = cpml_apply(dsxy, g.mem_y + 0 * sl.nys + sl.ry + k * sl.sy, ay, by, ky);
-            if (zs >= 0)
-                dsxz = cpml_apply(dsxz, g.mem_z + 0 * sl.nzs + zs * sl.sz + pc, g.pz.a[k], g.pz.b[k], g.pz.k[k]);
-            g.vx[p] += (dsxx + dsxz + dsxy) * dt / g.rho[p];
+            float dsxx = diff_fm<N>(P.sxx, c) / dx;
+            float dsxy = diff_bm<N>(P.sxy_y, c) / dy;
+            float dsxz = diff_bm<N>(wxz, c) / dz;
+            if (sl.xs >= 0) {
+                dsxx = cpml_step(dsxx, P.mx[0], cx.ah, cx.bh, cx.kh);
+                st(R.mx, 0 * sl.nxs + ox, P.mx[0]);
+            }
+            if (sl.ys >= 0) {
+                dsxy = cpml_step(dsxy, P.my[0], cy.a, cy.b, cy.k);
+                st(R.my, 0 * sl.nys + oy, P.my[0]);
+            }
+            if (P.zs >= 0) {
+                dsxz = cpml_step(dsxz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
+                st(R.mz, 0 * sl.nzs + oz, P.mz[0]);
+            }
+            st(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
         }
         // vy at (i, j+1/2, k): :180-195
         if (vy_col && k >= N + 1) {
-            const float rho_half_y = 0.5f * (g.rho[p] + g.rho[p + SY]);
-            float dsxy = d_bm<N>(g.sxy, p, 1, c) / dx;
-            float dsyy = d_fm<N>(g.syy, p, SY, c) / dy;
-            float dsyz = win_bm<N>(wyz, c) / dz;
-            if (sl.xs >= 0) dsxy = cpml_apply(dsxy, g.mem_x + 1 * sl.nxs + sl.rx + k * sl.sx, ax, bx, kx);
-            if (sl.ys >= 0) dsyy = cpml_apply(dsyy, g.mem_y + 1 * sl.nys + sl.ry + k * sl.sy, ay_h, by_h, ky_h);
-            if (zs >= 0)
-                dsyz = cpml_apply(dsyz, g.mem_z + 1 * sl.nzs + zs * sl.sz + pc, g.pz.a[k], g.pz.b[k], g.pz.k[k]);
-            g.vy[p] += (dsxy + dsyy + dsyz) * dt / rho_half_y;
+            const float rho_half_y = 0.5f * (P.rho[0] + P.rho[1]);
+            float dsxy = diff_bm<N>(P.sxy_x, c) / dx;
+            float dsyy = diff_fm<N>(P.syy, c) / dy;
+            float dsyz = diff_bm<N>(wyz, c) / dz;
+            if (sl.xs >= 0) {
+                dsxy = cpml_step(dsxy, P.mx[1], cx.a, cx.b, cx.k);
+                st(R.mx, 1 * sl.nxs + ox, P.mx[1]);
+            }
+            if (sl.ys >= 0) {
+                dsyy = cpml_step(dsyy, P.my[1], cy.ah, cy.bh, cy.kh);
+                st(R.my, 1 * sl.nys + oy, P.my[1]);
+            }
+            if (P.zs >= 0) {
+                dsyz = cpml_step(dsyz, P.mz[1], P.cz.a, P.cz.b, P.cz.k);
+                st(R.mz, 1 * sl.nzs + oz, P.mz[1]);
+            }
+            st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
         }
         // vz at (i, j, k+1/2): :197-212 (Tensor :228-243)
         if (vz_col && k < nz - N - 1) {
-            const float rho_avg = 0.25f * (g.rho[p] + g.rho[p + SY] + g.rho[p + 1 + SY] + g.rho[p + 1]);
-            float dsxz = d_bm<N>(g.sxz, p, 1, c) / dx;
-            float dsyz = d_bm<N>(g.syz, p, SY, c) / dy;
-            float dszz = win_fm<N>(wzz, c) / dz;
-            if (sl.xs >= 0) dsxz = cpml_apply(dsxz, g.mem_x + 2 * sl.nxs + sl.rx + k * sl.sx, ax, bx, kx);
-            if (sl.ys >= 0) dsyz = cpml_apply(dsyz, g.mem_y + 2 * sl.nys + sl.ry + k * sl.sy, ay, by, ky);
-            if (zs >= 0)
-                dszz = cpml_apply(dszz, g.mem_z + 2 * sl.nzs + zs * sl.sz + pc, g.pz.a_h[k], g.pz.b_h[k],
-                                  g.pz.k_h[k]);
+            const float rho_avg = 0.25f * (P.rho[0] + P.rho[1] + P.rho[2] + P.rho[3]);
+            float dsxz = diff_bm<N>(P.sxz_x, c) / dx;
+            float dsyz = diff_bm<N>(P.syz_y, c) / dy;
+            float dszz = diff_fm<N>(wzz, c) / dz;
+            if (sl.xs >= 0) {
+                dsxz = cpml_step(dsxz, P.mx[2], cx.a, cx.b, cx.k);
+                st(R.mx, 2 * sl.nxs + ox, P.mx[2]);
+            }
+            if (sl.ys >= 0) {
+                dsyz = cpml_step(dsyz, P.my[2], cy.a, cy.b, cy.k);
+                st(R.my, 2 * sl.nys + oy, P.my[2]);
+            }
+            if (P.zs >= 0) {
+                dszz = cpml_step(dszz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
+                st(R.mz, 2 * sl.nzs + oz, P.mz[2]);
+            }
             if (CLS == SW_CLASS_TENSOR)
-                g.vz[p] += (dsxz + dsyz + dszz) * dt / rho_avg;
+                st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt / rho_avg);
             else
-                g.vz[p] += (dszz + dsxz + dsyz) * dt / rho_avg;
+                st(R.vz, p, P.v[2] + (dszz + dsxz + dsyz) * dt / rho_avg);
         }
         win_shift<N>(wxz);
         win_shift<N>(wyz);
         win_shift<N>(wzz);
+#if SW_PREFETCH
+        P = Q;
+#endif
+    }
+}
+
+// -------------------------------------------------------------- stress sweep
+template <int N>
+struct SPlane {
+    float vx_x[2 * N];  // bm along x (Dx3bm, normal)
+    float vx_y[2 * N];  // fm along y (Dy3fm, sxy)
+    float vy_x[2 * N];  // fm along x (Dx3fm, sxy)
+    float vy_y[2 * N];  // bm along y (Dy3bm, normal)
+    float vz_x[2 * N];  // fm along x (Dx3fm, sxz)
+    float vz_y[2 * N];  // fm along y (Dy3fm, syz)
+    float lam[2];       // p, p+1
+    float mu[4];        // p, p+1, p+SY, p+SZ
+    float s[6];         // sxx syy szz sxy sxz syz at p
+    float wvx, wvy, wvz; // next window entries: vx, vy at k+N; vz at k+N-1
+    float mx[3], my[3], mz[3];
+    Coef cz;
+    int zs;
+};
+
+template <int N>
+__device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
+                                            const Slab &sl)
+{
+    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
+    const int c0 = N * 4 + N * SY;
+    taps_bm<N>(P.vx_x, R.vx, vb, c0, 4);
+    taps_fm<N>(P.vx_y, R.vx, vb, c0, SY);
+    taps_fm<N>(P.vy_x, R.vy, vb, c0, 4);
+    taps_bm<N>(P.vy_y, R.vy, vb, c0, SY);
+    taps_fm<N>(P.vz_x, R.vz, vb, c0, 4);
+    taps_fm<N>(P.vz_y, R.vz, vb, c0, SY);
+    P.lam[0] = ldu(R.lam, vb, c0);
+    P.lam[1] = ldu(R.lam, vb, c0 + 4);
+    P.mu[0] = ldu(R.mu, vb, c0);
+    P.mu[1] = ldu(R.mu, vb, c0 + 4);
+    P.mu[2] = ldu(R.mu, vb, c0 + SY);
+    P.mu[3] = ldu(R.mu, vb, c0 + (int)SZ);
+    P.s[0] = ldu(R.sxx, vb, c0);
+    P.s[1] = ldu(R.syy, vb, c0);
+    P.s[2] = ldu(R.szz, vb, c0);
+    P.s[3] = ldu(R.sxy, vb, c0);
+    P.s[4] = ldu(R.sxz, vb, c0);
+    P.s[5] = ldu(R.syz, vb, c0);
+    P.wvx = ldu(R.vx, vb, c0 + N * (int)SZ);
+    P.wvy = ldu(R.vy, vb, c0 + N * (int)SZ);
+    P.wvz = ldu(R.vz, vb, c0 + (N - 1) * (int)SZ);
+    P.zs = slab_index(k, g.pz.lo, g.pz.hi);
+    P.cz = coef_of(g.pz, k, P.zs >= 0);
+    const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+    const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        // x: dvx_dx, dvy_dx, dvz_dx; y: dvy_dy, dvx_dy, dvz_dy; z: dvz_dz, dvx_dz, dvy_dz
+        P.mx[m] = sl.xs >= 0 ? ldu(R.mx, ox, (int)((3 + m) * sl.nxs)) : 0.f;
+        P.my[m] = sl.ys >= 0 ? ldu(R.my, oy, (int)((3 + m) * sl.nys)) : 0.f;
+        P.mz[m] = P.zs >= 0 ? ldu(R.mz, oz, (int)((3 + m) * sl.nzs)) : 0.f;
     }
 }
 
@@ -202,7 +438,8 @@ __global__ void __launch_bounds__(TX *TY) k_stress(sw_ssg3d g, Geo geo)
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (i < N || i >= nx - N || j < N || j >= ny - N) return;
-    const long SY = nx, SZ = (long)nx * ny;
+    const int SY = 4 * nx;
+    const uint32_t SZ = 4u * (uint32_t)(nx * ny);
     const float dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
     float c[N];
 #pragma unroll
@@ -211,87 +448,122 @@ __global__ void __launch_bounds__(TX *TY) k_stress(sw_ssg3d g, Geo geo)
     const bool sxy_col = i >= N + 1 && j >= N + 1;
     const bool sxz_col = i < nx - N - 1;
     const bool syz_col = j < ny - N - 1;
-    const SlabIdx sl = slab_setup(g, i, j);
-    float ax_h = 0.f, bx_h = 1.f, kx_h = 1.f, ax = 0.f, bx = 1.f, kx = 1.f;
-    float ay_h = 0.f, by_h = 1.f, ky_h = 1.f, ay = 0.f, by = 1.f, ky = 1.f;
-    if (sl.xs >= 0) {
-        ax_h = g.px.a_h[i]; bx_h = g.px.b_h[i]; kx_h = g.px.k_h[i];
-        ax = g.px.a[i]; bx = g.px.b[i]; kx = g.px.k[i];
-    }
-    if (sl.ys >= 0) {
-        ay_h = g.py.a_h[j]; by_h = g.py.b_h[j]; ky_h = g.py.k_h[j];
-        ay = g.py.a[j]; by = g.py.b[j]; ky = g.py.k[j];
-    }
-    const long pc = (long)j * SY + i;
+    const Slab sl = slab_setup(g, i, j);
+    const Res R = make_res(g);
+    const Coef cx = coef_of(g.px, i, sl.xs >= 0);
+    const Coef cy = coef_of(g.py, j, sl.ys >= 0);
+    const uint32_t pc = 4u * (uint32_t)(j * nx + i);
     // windows: wvz[t] = vz[k-N+t] (backward), wvx[t] = vx[k-N+1+t], wvy[t] = vy[k-N+1+t] (forward)
     float wvz[2 * N], wvx[2 * N], wvy[2 * N];
     const int k0 = cl.k0;
 #pragma unroll
     for (int t = 0; t < 2 * N - 1; ++t) {
-        wvz[t] = g.vz[pc + (long)(k0 - N + t) * SZ];
-        wvx[t] = g.vx[pc + (long)(k0 - N + 1 + t) * SZ];
-        wvy[t] = g.vy[pc + (long)(k0 - N + 1 + t) * SZ];
+        wvz[t] = ld(R.vz, pc + (uint32_t)(k0 - N + t) * SZ);
+        wvx[t] = ld(R.vx, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
+        wvy[t] = ld(R.vy, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
     }
+    SPlane<N> P;
+#if SW_PREFETCH
+    load_splane<N>(P, g, R, pc, k0, SY, SZ, sl);
+#endif
     for (int k = k0; k < cl.k1; ++k) {
-        const long p = pc + (long)k * SZ;
-        wvz[2 * N - 1] = g.vz[p + (long)(N - 1) * SZ];
-        wvx[2 * N - 1] = g.vx[p + (long)N * SZ];
-        wvy[2 * N - 1] = g.vy[p + (long)N * SZ];
-        const int zs = slab_index(k, g.pz.lo, g.pz.hi);
+#if SW_PREFETCH
+        SPlane<N> Q;
+        if (k + 1 < cl.k1) load_splane<N>(Q, g, R, pc, k + 1, SY, SZ, sl);
+#else
+        load_splane<N>(P, g, R, pc, k, SY, SZ, sl);
+#endif
+        const uint32_t p = pc + (uint32_t)k * SZ;
+        const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+        const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+        wvz[2 * N - 1] = P.wvz;
+        wvx[2 * N - 1] = P.wvx;
+        wvy[2 * N - 1] = P.wvy;
         // normal stresses at (i, j, k): src/Elastic3D_SSG_CPML.py:247-266
         if (nrm_col && k >= N + 1) {
-            const float lam_h = 0.5f * (g.lam[p] + g.lam[p + 1]);
-            const float mu_h = 0.5f * (g.mu[p] + g.mu[p + 1]);
+            const float lam_h = 0.5f * (P.lam[0] + P.lam[1]);
+            const float mu_h = 0.5f * (P.mu[0] + P.mu[1]);
             const float lpm = lam_h + 2.f * mu_h;
-            float dvxx = d_bm<N>(g.vx, p, 1, c) / dx;
-            float dvyy = d_bm<N>(g.vy, p, SY, c) / dy;
-            float dvzz = win_bm<N>(wvz, c) / dz;
-            if (sl.xs >= 0) dvxx = cpml_apply(dvxx, g.mem_x + 3 * sl.nxs + sl.rx + k * sl.sx, ax, bx, kx);
-            if (sl.ys >= 0) dvyy = cpml_apply(dvyy, g.mem_y + 3 * sl.nys + sl.ry + k * sl.sy, ay, by, ky);
-            if (zs >= 0)
-                dvzz = cpml_apply(dvzz, g.mem_z + 3 * sl.nzs + zs * sl.sz + pc, g.pz.a[k], g.pz.b[k], g.pz.k[k]);
-            g.sxx[p] += (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt;
-            g.syy[p] += (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt;
-            g.szz[p] += (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt;
+            float dvxx = diff_bm<N>(P.vx_x, c) / dx;
+            float dvyy = diff_bm<N>(P.vy_y, c) / dy;
+            float dvzz = diff_bm<N>(wvz, c) / dz;
+            if (sl.xs >= 0) {
+                dvxx = cpml_step(dvxx, P.mx[0], cx.a, cx.b, cx.k);
+                st(R.mx, 3 * sl.nxs + ox, P.mx[0]);
+            }
+            if (sl.ys >= 0) {
+                dvyy = cpml_step(dvyy, P.my[0], cy.a, cy.b, cy.k);
+                st(R.my, 3 * sl.nys + oy, P.my[0]);
+            }
+            if (P.zs >= 0) {
+                dvzz = cpml_step(dvzz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
+                st(R.mz, 3 * sl.nzs + oz, P.mz[0]);
+            }
+            st(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
+            st(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
+            st(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
         }
         // sxy at (i+1/2, j+1/2, k): :267-279
         if (sxy_col) {
-            const float mu_a = 0.25f * (g.mu[p + SY] + g.mu[p] + g.mu[p] + g.mu[p + 1]);
-            float dvyx = d_fm<N>(g.vy, p, 1, c) / dx;
-            float dvxy = d_fm<N>(g.vx, p, SY, c) / dy;
-            if (sl.xs >= 0) dvyx = cpml_apply(dvyx, g.mem_x + 4 * sl.nxs + sl.rx + k * sl.sx, ax_h, bx_h, kx_h);
-            if (sl.ys >= 0) dvxy = cpml_apply(dvxy, g.mem_y + 4 * sl.nys + sl.ry + k * sl.sy, ay_h, by_h, ky_h);
-            g.sxy[p] += mu_a * (dvxy + dvyx) * dt;
+            const float mu_a = 0.25f * (P.mu[2] + P.mu[0] + P.mu[0] + P.mu[1]);
+            float dvyx = diff_fm<N>(P.vy_x, c) / dx;
+            float dvxy = diff_fm<N>(P.vx_y, c) / dy;
+            if (sl.xs >= 0) {
+                dvyx = cpml_step(dvyx, P.mx[1], cx.ah, cx.bh, cx.kh);
+                st(R.mx, 4 * sl.nxs + ox, P.mx[1]);
+            }
+            if (sl.ys >= 0) {
+                dvxy = cpml_step(dvxy, P.my[1], cy.ah, cy.bh, cy.kh);
+                st(R.my, 4 * sl.nys + oy, P.my[1]);
+            }
+            st(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
         }
         // sxz at (i+1/2, j, k+1/2): :281-293
         if (sxz_col && k < nz - N - 1) {
-            const float mu_a = 0.25f * (g.mu[p] + g.mu[p + 1] + g.mu[p] + g.mu[p + SY]);
-            float dvzx = d_fm<N>(g.vz, p, 1, c) / dx;
-            float dvxz = win_fm<N>(wvx, c) / dz;
-            if (sl.xs >= 0) dvzx = cpml_apply(dvzx, g.mem_x + 5 * sl.nxs + sl.rx + k * sl.sx, ax_h, bx_h, kx_h);
-            if (zs >= 0)
-                dvxz = cpml_apply(dvxz, g.mem_z + 4 * sl.nzs + zs * sl.sz + pc, g.pz.a_h[k], g.pz.b_h[k],
-                                  g.pz.k_h[k]);
-            g.sxz[p] += mu_a * (dvxz + dvzx) * dt;
+            const float mu_a = 0.25f * (P.mu[0] + P.mu[1] + P.mu[0] + P.mu[2]);
+            float dvzx = diff_fm<N>(P.vz_x, c) / dx;
+            float dvxz = diff_fm<N>(wvx, c) / dz;
+            if (sl.xs >= 0) {
+                dvzx = cpml_step(dvzx, P.mx[2], cx.ah, cx.bh, cx.kh);
+                st(R.mx, 5 * sl.nxs + ox, P.mx[2]);
+            }
+            if (P.zs >= 0) {
+                dvxz = cpml_step(dvxz, P.mz[1], P.cz.ah, P.cz.bh, P.cz.kh);
+                st(R.mz, 4 * sl.nzs + oz, P.mz[1]);
+            }
+            st(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
         }
         // syz at (i, j+1/2, k+1/2): :295-307
         if (syz_col && k < nz - N - 1) {
-            const float mu_a = 0.25f * (g.mu[p + SY] + g.mu[p] + g.mu[p + SZ] + g.mu[p]);
-            float dvyz = win_fm<N>(wvy, c) / dz;
-            float dvzy = d_fm<N>(g.vz, p, SY, c) / dy;
-            if (sl.ys >= 0) dvzy = cpml_apply(dvzy, g.mem_y + 5 * sl.nys + sl.ry + k * sl.sy, ay_h, by_h, ky_h);
-            if (zs >= 0)
-                dvyz = cpml_apply(dvyz, g.mem_z + 5 * sl.nzs + zs * sl.sz + pc, g.pz.a_h[k], g.pz.b_h[k],
-                                  g.pz.k_h[k]);
-            g.syz[p] += mu_a * (dvyz + dvzy) * dt;
+            const float mu_a = 0.25f * (P.mu[2] + P.mu[0] + P.mu[3] + P.mu[0]);
+            float dvyz = diff_fm<N>(wvy, c) / dz;
+            float dvzy = diff_fm<N>(P.vz_y, c) / dy;
+            if (sl.ys >= 0) {
+                dvzy = cpml_step(dvzy, P.my[2], cy.ah, cy.bh, cy.kh);
+                st(R.my, 5 * sl.nys + oy, P.my[2]);
+            }
+            if (P.zs >= 0) {
+                dvyz = cpml_step(dvyz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
+                st(R.mz, 5 * sl.nzs + oz, P.mz[2]);
+            }
+            st(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
         }
         win_shift<N>(wvz);
         win_shift<N>(wvx);
         win_shift<N>(wvy);
+#if SW_PREFETCH
+        P = Q;
+#endif
     }
 }
 
-constexpr int kTX = 64, kTY = 4;
+#ifndef SW_TY
+#define SW_TY 4
+#endif
+#ifndef SW_CZ0
+#define SW_CZ0 8
+#endif
+constexpr int kTX = 64, kTY = SW_TY;
 
 Geo make_geo(const sw_ssg3d &g, int N, int cz)
 {
@@ -309,7 +581,7 @@ int choose_cz(const sw_ssg3d &g, int N)
 {
     const int cols = ((g.nx + kTX - 1) / kTX) * ((g.ny + kTY - 1) / kTY);
     const int nzi = g.nz - 2 * N;
-    int cz = 16;
+    int cz = SW_CZ0;
     while (cz < 64 && (long)cols * ((nzi + 2 * cz - 1) / (2 * cz)) >= 4096) cz *= 2;
     return cz > nzi ? nzi : cz;
 }

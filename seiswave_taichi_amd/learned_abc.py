@@ -1,0 +1,350 @@
+"""Learned thin absorbing boundary (no counterpart in the reference).
+
+A thin CPML (4 cells) plus one small MLP per face that corrects the fields of
+the 4 outermost interior cells after every step, replacing most of a wide
+(20-cell) CPML.  The networks run in the fused HIP kernel sw_band_correct;
+training uses the HIP dense forward/backward/Adam kernels (sw_mlp_*), on
+samples generated by the plain solver itself.
+
+Training targets (teacher forcing).  A wide-CPML run of a random model is the
+reference.  A thin-CPML run of the same interior is reset to the reference's
+interior state before every step; after the step, the features are the thin
+run's band (9 fields x 4 cells) and the band before the step, and the target
+is the reference band minus the thin band: the one-step error the thin
+boundary makes next to itself.  At inference the corrected band is fed back as
+the "previous" half of the features, as in training.
+
+Configuration: configs' 9 field components imply 3-D, so the training runs are
+small 3-D boxes (layered, perturbed models, random moment-tensor sources) and
+the evaluation is the 200^3 homogeneous case against a 20-cell CPML.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .coeffs import ricker2
+
+DEPTH, NFEAT, NOUT, NH = 4, 72, 36, 64
+FACES = ("x-lo", "x-hi", "y-lo", "y-hi", "z-lo", "z-hi")
+
+
+SwBand = L.SwBand
+
+
+def _sig():
+    return L.load()
+
+
+def param_count() -> int:
+    return int(_sig().sw_mlp_param_count())
+
+
+def layout():
+    """Offsets of (W1, b1, W2, b2, W3, b3, in_scale, out_scale) in one face's parameters."""
+    sizes = [NFEAT * NH, NH, NH * NH, NH, NH * NOUT, NOUT, NFEAT, NOUT]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    names = ("W1", "b1", "W2", "b2", "W3", "b3", "in_scale", "out_scale")
+    return {n: (int(off[i]), int(off[i + 1])) for i, n in enumerate(names)}
+
+
+def band_of(engine) -> SwBand:
+    """Interior box (outside every CPML slab) of a 3-D engine."""
+    b = SwBand()
+    (ax, ay, az) = engine.pml
+    b.ox, b.oy, b.oz = ax.lo, ay.lo, az.lo
+    b.ex, b.ey, b.ez = ax.hi - ax.lo, ay.hi - ay.lo, az.hi - az.lo
+    return b
+
+
+def face_columns(b: SwBand):
+    """Number of columns of each face and their start offsets."""
+    ex2, ey2 = b.ex - 2 * DEPTH, b.ey - 2 * DEPTH
+    n = [b.ey * b.ez, b.ey * b.ez, ex2 * b.ez, ex2 * b.ez, ex2 * ey2, ex2 * ey2]
+    return n, np.concatenate([[0], np.cumsum(n)])
+
+
+class BandNet:
+    """Parameters of the six face networks, one device tensor [6, P]."""
+
+    def __init__(self, device=None, seed=0):
+        self.P = param_count()
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.lay = layout()
+        self.params = torch.zeros((6, self.P), dtype=torch.float32, device=self.device)
+        rng = np.random.default_rng(seed)
+        host = np.zeros((6, self.P), np.float32)
+        for f in range(6):
+            for name, (fan_in, fan_out) in (("W1", (NFEAT, NH)), ("W2", (NH, NH)), ("W3", (NH, NOUT))):
+                a, b = self.lay[name]
+                lim = np.sqrt(6.0 / (fan_in + fan_out))
+                host[f, a:b] = rng.uniform(-lim, lim, b - a)
+            a, b = self.lay["W3"]
+            host[f, a:b] *= 0.1  # start close to "no correction"
+            for name in ("in_scale", "out_scale"):
+                a, b = self.lay[name]
+                host[f, a:b] = 1.0
+        self.params.copy_(torch.from_numpy(host))
+
+    def set_scales(self, face, in_scale, out_scale):
+        a, b = self.lay["in_scale"]
+        self.params[face, a:b] = torch.as_tensor(in_scale, dtype=torch.float32, device=self.device)
+        a, b = self.lay["out_scale"]
+        self.params[face, a:b] = torch.as_tensor(out_scale, dtype=torch.float32, device=self.device)
+
+    def save(self, path):
+        np.savez(path, params=self.params.cpu().numpy())
+
+    @classmethod
+    def load(cls, path, device=None):
+        net = cls(device)
+        with np.load(path, allow_pickle=False) as z:
+            net.params.copy_(torch.from_numpy(z["params"]))
+        return net
+
+
+# ------------------------------------------------------------ data generation
+def random_layered_model(shape, rng, vp_range=(2000.0, 4500.0), nlayers=4, pert=0.05):
+    """Random layered model along the last axis with a smooth random
+    perturbation; vs = vp/1.73, Gardner density."""
+    from scipy import ndimage
+    nz = shape[-1]
+    cuts = np.sort(rng.integers(1, nz - 1, nlayers - 1))
+    vals = np.sort(rng.uniform(*vp_range, nlayers))
+    prof = np.empty(nz)
+    edges = np.concatenate([[0], cuts, [nz]])
+    for t in range(nlayers):
+        prof[edges[t]:edges[t + 1]] = vals[t]
+    vp = np.broadcast_to(prof, shape).astype(np.float64)
+    noise = ndimage.gaussian_filter(rng.standard_normal(shape), 3.0, mode="wrap")
+    noise /= noise.std() + 1e-12
+    vp = vp * (1 + pert * noise)
+    vs = vp / 1.73
+    rho = 1741.0 * (vp / 1000.0) ** 0.25
+    return vs.astype(np.float32), vp.astype(np.float32), rho.astype(np.float32)
+
+
+def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
+    from .Elastic3D_SSG_Tensor import ElasticWAVE
+    nt = len(amps)
+    mt3 = np.zeros((3, 3), np.float32)
+    w = ElasticWAVE(vs, vp, rho, d, d, d, dt, *cell, np.zeros((1, 1), np.int64) + cell[0],
+                    np.zeros((1, 1), np.int64) + cell[1], np.zeros((1, 1), np.int64) + cell[2], mt3, 1.0, nt, 2, f,
+                    source={"kind": "moment", "cell": cell, "mt": mt, "wavelet": amps})
+    p = {"vp_max": float(vp.max()), "pml_x_thick": npml * d, "pml_y_thick": npml * d, "pml_z_thick": npml * d,
+         "Rcoef": 1e-6, "theta": 1, "alpha_max_pml": np.pi * f, "kmax_pml": 1.0}
+    w.SetADEPML3D([True] * 6, p)
+    return w
+
+
+def grid_size(n_int, npml, s=2):
+    """Grid points per axis that leave n_int interior cells with an npml-cell CPML."""
+    return n_int + 2 * s + 2 * npml + 1
+
+
+def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20, keep=0.2):
+    """One teacher-forced run: returns per-face (X [m,72], T [m,36]) sample lists."""
+    lib = _sig()
+    nw, nt_ = grid_size(n_int, wide), grid_size(n_int, thin)
+    off = wide - thin
+    vs, vp, rho = random_layered_model((nw, nw, nw), rng)
+    lo_w = 2 + wide
+    src = tuple(int(v) for v in rng.integers(lo_w + 6, lo_w + n_int - 6, 3))
+    m = rng.standard_normal(6)
+    m /= np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2 + 2 * (m[3] ** 2 + m[4] ** 2 + m[5] ** 2))
+    amps = (1e6 * ricker2(np.arange(steps), dt, 1.2 / f, f, 1.0)).astype(np.float32)
+    W = _sim(vs, vp, rho, d, dt, src, m, amps, wide, f)
+    sl = slice(off, off + nt_)
+    T_ = _sim(vs[sl, sl, sl].copy(), vp[sl, sl, sl].copy(), rho[sl, sl, sl].copy(), d, dt,
+              tuple(c - off for c in src), m, amps, thin, f)
+    bt, bw = band_of(T_._e), band_of(W._e)
+    ncol = lib.sw_band_columns(C.byref(bt))
+    counts, col0 = face_columns(bt)
+    dev = T_._e.device
+    cur = torch.empty((ncol, NOUT), device=dev)
+    prv = torch.empty((ncol, NOUT), device=dev)
+    wcur = torch.empty((ncol, NOUT), device=dev)
+    st = T_._e.stream()
+    xs = {f_: [] for f_ in range(6)}
+    ts = {f_: [] for f_ in range(6)}
+    names = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    # interior boxes in device layout [z][y][x]
+    ti = (slice(bt.oz, bt.oz + bt.ez), slice(bt.oy, bt.oy + bt.ey), slice(bt.ox, bt.ox + bt.ex))
+    wi = (slice(bw.oz, bw.oz + bw.ez), slice(bw.oy, bw.oy + bw.ey), slice(bw.ox, bw.ox + bw.ex))
+    for n in range(steps):
+        for nm in names:
+            T_._e.f[nm][ti].copy_(W._e.f[nm][wi])
+        gt, gw = T_._e.descriptor(), W._e.descriptor()
+        L.check(lib.sw_band_gather(gt, C.byref(bt), L.fptr(prv), st), "band gather")
+        T_.run(1, n)
+        W.run(1, n)
+        gt, gw = T_._e.descriptor(), W._e.descriptor()
+        L.check(lib.sw_band_gather(gt, C.byref(bt), L.fptr(cur), st), "band gather")
+        L.check(lib.sw_band_gather(gw, C.byref(bw), L.fptr(wcur), st), "band gather")
+        if n < 10:
+            continue
+        x = torch.cat([cur, prv], 1)
+        t = wcur - cur
+        for f_ in range(6):
+            a, b = int(col0[f_]), int(col0[f_ + 1])
+            k = max(1, int(keep * (b - a)))
+            idx = torch.randint(a, b, (k,), device=dev)
+            xs[f_].append(x[idx])
+            ts[f_].append(t[idx])
+    return {f_: (torch.cat(xs[f_]), torch.cat(ts[f_])) for f_ in range(6)}
+
+
+def train_face(net: BandNet, face, X, T, iters=3000, batch=2048, lr=2e-3, seed=0, log=None):
+    """Fit one face network with the HIP training kernels; returns the loss history."""
+    lib = _sig()
+    dev = net.device
+    # normalisation: RMS per input channel / per output channel
+    in_scale = torch.sqrt(torch.mean(X * X, 0)) + 1e-20
+    out_scale = torch.sqrt(torch.mean(T * T, 0)) + 1e-20
+    net.set_scales(face, in_scale, out_scale)
+    p = net.params[face]
+    ntrain = net.lay["b3"][1]  # weights and biases come first
+    grad = torch.zeros(ntrain, device=dev)
+    m1 = torch.zeros(ntrain, device=dev)
+    m2 = torch.zeros(ntrain, device=dev)
+    H1 = torch.empty((batch, NH), device=dev)
+    H2 = torch.empty((batch, NH), device=dev)
+    Y = torch.empty((batch, NOUT), device=dev)
+    work = torch.empty(batch * (NOUT + 2 * NH), device=dev)
+    loss_rows = torch.empty(batch, device=dev)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    st = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    hist = []
+    n = X.shape[0]
+    for it in range(1, iters + 1):
+        idx = torch.randint(0, n, (batch,), device=dev, generator=g)
+        xb = X[idx].contiguous()
+        tb = T[idx].contiguous()
+        L.check(lib.sw_mlp_forward(L.fptr(p), L.fptr(xb), batch, L.fptr(H1), L.fptr(H2), L.fptr(Y), st), "mlp fwd")
+        L.check(lib.sw_mlp_loss_backward(L.fptr(p), L.fptr(xb), L.fptr(H1), L.fptr(H2), L.fptr(Y), L.fptr(tb),
+                                         batch, L.fptr(grad), L.fptr(work), L.fptr(loss_rows), st), "mlp bwd")
+        cur_lr = lr * (0.1 ** (it / iters))
+        L.check(lib.sw_adam(L.fptr(p), L.fptr(grad), L.fptr(m1), L.fptr(m2), ntrain, cur_lr, 0.9, 0.999, 1e-8,
+                            it, st), "adam")
+        if it % 100 == 0 or it == 1:
+            loss = float(loss_rows.sum().item()) / (batch * NOUT)
+            hist.append((it, loss))
+            if log:
+                log(f"face {FACES[face]} iter {it} loss {loss:.4e}")
+    return hist
+
+
+def train(n_runs=16, n_int=40, steps=160, iters=3000, seed=0, log=print):
+    """Generate teacher-forced samples with the plain solver and train all faces."""
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    pools = {f: ([], []) for f in range(6)}
+    for r in range(n_runs):
+        data = collect_run(rng, n_int=n_int, steps=steps)
+        for f in range(6):
+            pools[f][0].append(data[f][0])
+            pools[f][1].append(data[f][1])
+        if log:
+            log(f"run {r + 1}/{n_runs} collected ({time.time() - t0:.1f} s)")
+    net = BandNet(seed=seed)
+    hists = {}
+    for f in range(6):
+        X = torch.cat(pools[f][0])
+        T = torch.cat(pools[f][1])
+        hists[f] = train_face(net, f, X, T, iters=iters, seed=seed + f, log=log)
+    return net, hists
+
+
+# ---------------------------------------------------------------- inference
+class LearnedRun:
+    """Runs an ElasticWAVE (3-D) instance with the learned band correction."""
+
+    def __init__(self, sim, net: BandNet):
+        self.lib = _sig()
+        self.sim = sim
+        self.net = net
+        self.band = band_of(sim._e)
+        ncol = self.lib.sw_band_columns(C.byref(self.band))
+        self.prev = torch.zeros((ncol, NOUT), device=sim._e.device)
+
+    def reset(self):
+        self.prev.zero_()
+
+    def run(self, nsteps, it0=0):
+        e = self.sim._e
+        g = e.descriptor()
+        L.check(self.lib.sw_ssg3d_run_learned(g, self.sim._src.s, self.sim._rec.r, C.byref(self.band),
+                                              L.fptr(self.net.params), L.fptr(self.prev), it0, nsteps,
+                                              e.stream()), "sw_ssg3d_run_learned")
+
+
+# --------------------------------------------------------------- evaluation
+def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20,
+                     model=None, src=None, every=5, timing_steps=0):
+    """Reflected energy of thin CPML (with and without the network) and of the
+    wide CPML, against a padded reference whose own boundary reflections
+    cannot reach the interior within ``steps``.
+
+    ``model`` is (vs, vp, rho) over the n_int^3 interior; every run extends it
+    outwards by edge replication.  Energy = sum over snapshots every ``every``
+    steps and over the interior box of (v - v_ref)^2 for the three velocity
+    components.  ``timing_steps`` > 0 also times the thin+network and the wide
+    runs (ms per step, HIP events)."""
+    rng = np.random.default_rng(seed)
+    if model is None:
+        model = random_layered_model((n_int,) * 3, rng)
+    vmax = float(np.max(model[1]))
+    pad_ref = int(np.ceil(vmax * dt * steps / d / 2)) + 4
+    if src is None:
+        src = tuple(int(v) for v in rng.integers(6, n_int - 6, 3))
+    m = rng.standard_normal(6)
+    m /= np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2 + 2 * (m[3] ** 2 + m[4] ** 2 + m[5] ** 2))
+    nt_all = steps + max(timing_steps, 0)
+    amps = (1e6 * ricker2(np.arange(nt_all), dt, 1.2 / f, f, 1.0)).astype(np.float32)
+    runs = {}
+    for name, npml, extra in (("ref", wide, pad_ref), ("wide", wide, 0), ("thin", thin, 0), ("learned", thin, 0)):
+        lo = 2 + npml + extra  # first interior index on every axis
+        mdl = tuple(np.ascontiguousarray(np.pad(np.asarray(a, np.float32), ((lo, lo + 1),) * 3, mode="edge"))
+                    for a in model)
+        sim = _sim(*mdl, d, dt, tuple(c + lo for c in src), m, amps, npml, f)
+        runs[name] = (sim, lo)
+    learned = LearnedRun(runs["learned"][0], net)
+    energy = {k: 0.0 for k in ("wide", "thin", "learned")}
+    ref_norm = 0.0
+    comps = ("vx", "vy", "vz")
+    for n0 in range(0, steps, every):
+        k = min(every, steps - n0)
+        for name, (sim, lo) in runs.items():
+            if name == "learned":
+                learned.run(k, n0)
+            else:
+                sim.run(k, n0)
+        rs, rlo = runs["ref"]
+        box = lambda lo: (slice(lo, lo + n_int),) * 3  # noqa: E731
+        for cmp in comps:
+            r = rs._e.f[cmp][box(rlo)]
+            ref_norm += float(torch.sum(r.double() ** 2))
+            for name in energy:
+                sim, lo = runs[name]
+                energy[name] += float(torch.sum((sim._e.f[cmp][box(lo)].double() - r.double()) ** 2))
+    out = dict(energy)
+    out["reference_energy"] = ref_norm
+    out["ratio_learned_to_wide"] = energy["learned"] / max(energy["wide"], 1e-300)
+    out["ratio_thin_to_wide"] = energy["thin"] / max(energy["wide"], 1e-300)
+    if timing_steps > 0:
+        for name in ("wide", "learned"):
+            sim, _ = runs[name]
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            if name == "learned":
+                learned.run(timing_steps, steps)
+            else:
+                sim.run(timing_steps, steps)
+            b.record()
+            torch.cuda.synchronize()
+            out[f"ms_per_step_{name}"] = a.elapsed_time(b) / timing_steps
+    return out

@@ -1,0 +1,160 @@
+"""Learned thin boundary: HIP MLP training kernels against the f64 oracle,
+band geometry, the fused correction kernel, and the gain of a trained
+network over the same thin CPML without it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _net_params(seed):
+    from seiswave_taichi_amd import learned_abc as LA
+    net = LA.BandNet(seed=seed)
+    rng = np.random.default_rng(seed)
+    lay = net.lay
+    host = net.params.cpu().numpy()
+    for f in range(6):
+        a, b = lay["in_scale"]
+        host[f, a:b] = rng.uniform(0.5, 2.0, b - a)
+        a, b = lay["out_scale"]
+        host[f, a:b] = rng.uniform(0.5, 2.0, b - a)
+        a, b = lay["b1"]
+        host[f, a:b] = 0.1 * rng.standard_normal(b - a)
+    net.params.copy_(torch.from_numpy(host))
+    return net
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd import learned_abc as LA
+    lib = LA._sig()
+    net = _net_params(1)
+    p = net.params[2].contiguous()
+    B = 300
+    rng = np.random.default_rng(2)
+    X = torch.from_numpy(rng.standard_normal((B, 72)).astype(np.float32)).cuda()
+    T = torch.from_numpy(rng.standard_normal((B, 36)).astype(np.float32)).cuda()
+    H1, H2 = torch.empty((B, 64), device="cuda"), torch.empty((B, 64), device="cuda")
+    Y = torch.empty((B, 36), device="cuda")
+    ntrain = net.lay["b3"][1]
+    grad = torch.empty(ntrain, device="cuda")
+    work = torch.empty(B * (36 + 128), device="cuda")
+    rows = torch.empty(B, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L.check(lib.sw_mlp_forward(L.fptr(p), L.fptr(X), B, L.fptr(H1), L.fptr(H2), L.fptr(Y), st), "fwd")
+    L.check(lib.sw_mlp_loss_backward(L.fptr(p), L.fptr(X), L.fptr(H1), L.fptr(H2), L.fptr(Y), L.fptr(T), B,
+                                     L.fptr(grad), L.fptr(work), L.fptr(rows), st), "bwd")
+    torch.cuda.synchronize()
+    pn = p.cpu().numpy().astype(np.float64)
+    _, _, Yr = M.forward(pn, X.cpu().numpy())
+    np.testing.assert_allclose(Y.cpu().numpy(), Yr, rtol=1e-4, atol=1e-5)
+    Lr, gr = M.loss_and_grad(pn, X.cpu().numpy(), T.cpu().numpy())
+    assert abs(rows.sum().item() / (B * 36) - Lr) <= 1e-5 * Lr
+    g = grad.cpu().numpy()
+    assert np.linalg.norm(g - gr) <= 1e-4 * np.linalg.norm(gr)
+
+
+def test_adam_matches_oracle():
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd import learned_abc as LA
+    lib = LA._sig()
+    rng = np.random.default_rng(3)
+    n = 1000
+    p0, g = rng.standard_normal(n), rng.standard_normal(n)
+    m0, v0 = 0.1 * rng.standard_normal(n), rng.uniform(0, 0.1, n)
+    P, G, Mt, V = (torch.from_numpy(a.astype(np.float32)).cuda() for a in (p0, g, m0, v0))
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L.check(lib.sw_adam(L.fptr(P), L.fptr(G), L.fptr(Mt), L.fptr(V), n, 1e-3, 0.9, 0.999, 1e-8, 5, st), "adam")
+    pr, mr, vr = M.adam(p0, g, m0, v0, 1e-3, 0.9, 0.999, 1e-8, 5)
+    np.testing.assert_allclose(P.cpu().numpy(), pr, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(Mt.cpu().numpy(), mr, rtol=1e-5, atol=1e-7)
+
+
+def _small_sim(n_int=20, thin=4):
+    from seiswave_taichi_amd import learned_abc as LA
+    n = LA.grid_size(n_int, thin)
+    vs, vp, rho = LA.random_layered_model((n, n, n), np.random.default_rng(5))
+    amps = np.ones(10, np.float32)
+    return LA._sim(vs, vp, rho, 10.0, 1e-3, (n // 2,) * 3, [1, 1, 1, 0, 0, 0], amps, thin, 15.0)
+
+
+def test_band_geometry_and_gather():
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd import learned_abc as LA
+    lib = LA._sig()
+    w = _small_sim()
+    e = w._e
+    b = LA.band_of(e)
+    assert (b.ox, b.ex) == (6, 20)
+    ncol = lib.sw_band_columns(C.byref(b))
+    counts, col0 = LA.face_columns(b)
+    assert ncol == col0[-1] == 2 * 20 * 20 + 2 * 12 * 20 + 2 * 12 * 12
+    names = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    nx, ny, nz = e.gridsize
+    for t, nm in enumerate(names):
+        kk, jj, ii = torch.meshgrid(torch.arange(nz), torch.arange(ny), torch.arange(nx), indexing="ij")
+        e.f[nm].copy_(((kk * ny + jj) * nx + ii).float().cuda() * 10 + t)
+    out = torch.empty((ncol, 36), device="cuda")
+    L.check(lib.sw_band_gather(e.descriptor(), C.byref(b), L.fptr(out), e.stream()), "gather")
+    got = out.cpu().numpy()
+    # every band cell appears exactly once, with the 9 fields in order
+    flat = (got[:, 0::9] // 10).astype(np.int64).ravel()
+    assert len(np.unique(flat)) == flat.size == ncol * 4
+    np.testing.assert_array_equal(got[:, 1::9] - got[:, 0::9], 1.0)
+    # the band covers exactly the interior cells within 4 of a face
+    cells = np.stack(np.unravel_index(flat, (nz, ny, nx)), 1)[:, ::-1]  # (i, j, k)
+    lo = np.array([b.ox, b.oy, b.oz])
+    ext = np.array([b.ex, b.ey, b.ez])
+    rel = cells - lo
+    assert np.all((rel >= 0) & (rel < ext))
+    depth = np.minimum(rel, ext - 1 - rel).min(1)
+    assert depth.max() == 3
+    full = np.stack(np.meshgrid(*(np.arange(n) for n in ext), indexing="ij"), -1).reshape(-1, 3)
+    assert (np.minimum(full, ext - 1 - full).min(1) < 4).sum() == ncol * 4
+
+
+def test_fused_correction_matches_oracle():
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd import learned_abc as LA
+    lib = LA._sig()
+    w = _small_sim()
+    e = w._e
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for nm in e.f:
+        e.f[nm].copy_(torch.randn(e.f[nm].shape, generator=g, device="cuda"))
+    b = LA.band_of(e)
+    ncol = lib.sw_band_columns(C.byref(b))
+    net = _net_params(7)
+    prev = torch.randn((ncol, 36), generator=g, device="cuda")
+    prev0 = prev.clone()
+    cur = torch.empty((ncol, 36), device="cuda")
+    L.check(lib.sw_band_gather(e.descriptor(), C.byref(b), L.fptr(cur), e.stream()), "gather")
+    L.check(lib.sw_band_correct(e.descriptor(), C.byref(b), L.fptr(net.params), L.fptr(prev), e.stream()), "fix")
+    after = torch.empty((ncol, 36), device="cuda")
+    L.check(lib.sw_band_gather(e.descriptor(), C.byref(b), L.fptr(after), e.stream()), "gather")
+    torch.cuda.synchronize()
+    counts, col0 = LA.face_columns(b)
+    X = torch.cat([cur, prev0], 1).cpu().numpy()
+    params = net.params.cpu().numpy()
+    for f in range(6):
+        a, z = int(col0[f]), int(col0[f + 1])
+        corr = M.correction(params[f], X[a:z])
+        np.testing.assert_allclose(after.cpu().numpy()[a:z], X[a:z, :36] + corr, rtol=2e-4, atol=2e-4)
+    np.testing.assert_array_equal(prev.cpu().numpy(), after.cpu().numpy())
+
+
+def test_trained_network_beats_plain_thin_cpml():
+    """Train on a few small runs, then compare reflected energy on a held-out
+    model: thin CPML with the network vs without it (same grid, same steps)."""
+    from seiswave_taichi_amd import learned_abc as LA
+    torch.manual_seed(0)
+    net, hists = LA.train(n_runs=4, n_int=28, steps=120, iters=600, seed=11, log=None)
+    for f, h in hists.items():
+        assert h[-1][1] < h[0][1], (f, h[0], h[-1])
+    res = LA.reflection_study(net, n_int=28, steps=150, seed=99)
+    assert res["learned"] < res["thin"], res
