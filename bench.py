@@ -113,13 +113,13 @@ def cpu_baseline(w, budget_s=15.0):
     O.set_threads(cores)
     rx, ry, rz = w["rec"]
     o = O.Oracle3D("tensor", w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], w["src"],
-                   np.stack([rx, ry, rz], 1), 64, 2, freq=w["f"], mt=w["mt"])
+                   np.stack([rx, ry, rz], 1), 1600, 2, freq=w["f"], mt=w["mt"])
     o.set_pml([True] * 6, w["pml"])
     o.run(2)  # warm-up
     t0 = time.perf_counter()
     o.step(2)
     t1 = time.perf_counter() - t0
-    nsteps = int(max(2, min(40, budget_s / max(t1, 1e-3))))
+    nsteps = int(max(2, min(1500, budget_s / max(t1, 1e-3))))
     t0 = time.perf_counter()
     o.run(nsteps, it0=3)
     el = time.perf_counter() - t0
@@ -130,19 +130,101 @@ def cpu_baseline(w, budget_s=15.0):
                       f"{cores} OpenMP threads ({el:.1f} s)"}
 
 
+def gradient_workload(args):
+    """Misfit gradient of the 2-D adjoint at full size: 800 x 300 cells, 10 m,
+    layered background with two +-10 % Gaussian vp/vs anomalies (150 m), 16
+    explosive 10 Hz Ricker shots and 400 receivers (vx, vz) just below the top
+    CPML, 2500 steps, checkpoint every 50 steps, L2 misfit against the data of
+    the anomalous model.  Times one forward+backward (seconds per gradient)
+    and checks a few cells against central differences of the same f32
+    forward (step 1e-3 relative)."""
+    import torch
+
+    from seiswave_taichi_amd.adjoint import Elastic2DModeling, seismograms
+    from seiswave_taichi_amd.models import cfl_dt, layered_2d_with_anomalies
+
+    torch.cuda.set_device(0)
+    nx, nz, d, nt, npml, f = 800, 300, 10.0, 2500, 20, 10.0
+    (vp0, vs0, rho), (vp1, vs1, _) = layered_2d_with_anomalies((nx, nz), d, seed=0)
+    vmax = float(vp1.max())
+    dt = cfl_dt(vmax, (d, d))
+    pml = {"vp_max": vmax, "pml_x_thick": npml * d, "pml_z_thick": npml * d, "Rcoef": 1e-6, "theta": 1,
+           "alpha_max_pml": np.pi * f, "kmax_pml": 1.0}
+    top = 2 + npml + 2
+    shots = [(int(x), top) for x in np.round(np.linspace(40, nx - 41, 16))]
+    rec = np.stack([np.round(np.linspace(30, nx - 31, 400)).astype(np.int64), np.full(400, top)], 1)
+    m = Elastic2DModeling((nx, nz), d, d, dt, nt, 2, pml, shots, rec, freq=f, checkpoint=50)
+    dobs, _ = m.model(torch.from_numpy(vp1), torch.from_numpy(vs1), torch.from_numpy(rho))
+    dobs = dobs.clone()
+    tv = [torch.from_numpy(a).cuda().requires_grad_(True) for a in (vp0, vs0, rho)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    data = seismograms(*tv, m)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    loss = 0.5 * torch.sum((data.double() - dobs.double()) ** 2)
+    loss.backward()
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t0
+    grads = [t.grad.double().cpu().numpy() for t in tv]
+    if not all(np.all(np.isfinite(g)) for g in grads):
+        raise RuntimeError("non-finite gradient")
+
+    def misfit(vp, vs, rh):
+        dd, _ = m.model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rh))
+        return 0.5 * float(torch.sum((dd.double() - dobs.double()) ** 2))
+
+    rng = np.random.default_rng(1)
+    lo, hi = 2 + npml + 8, nz - 2 - npml - 8
+    fd_rel = {}
+    for k, name in enumerate(("vp", "vs", "rho")):
+        base = [vp0, vs0, rho]
+        fd, ad = [], []
+        for _ in range(args.fd_cells):
+            ci, cj = int(rng.integers(2 + npml + 8, nx - 2 - npml - 8)), int(rng.integers(lo, hi))
+            h = 1e-3 * float(base[k][ci, cj])
+            plus = [a.copy() for a in base]
+            minus = [a.copy() for a in base]
+            plus[k][ci, cj] += h
+            minus[k][ci, cj] -= h
+            fd.append((misfit(*plus) - misfit(*minus)) / (2 * h))
+            ad.append(grads[k][ci, cj])
+        fd, ad = np.array(fd), np.array(ad)
+        fd_rel[name] = float(np.linalg.norm(ad - fd) / max(np.linalg.norm(fd), 1e-300))
+    cells = nx * nz * nt * len(shots)
+    out = {"metric": "2D adjoint gradient seconds", "value": round(t_all, 3), "unit": "s/gradient",
+           "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "cfg5_gradient: 800x300 2-D VS, 16 explosive shots, 400 receivers (vx, vz), "
+                                  "2500 steps, checkpoint 50, L2 misfit", "dt": dt},
+           "forward_s": round(t_fwd, 3), "backward_s": round(t_all - t_fwd, 3),
+           "forward_Gcells_per_s": round(cells / t_fwd / 1e9, 3),
+           "misfit": float(loss.item()),
+           "fd_check": {"cells_per_parameter": args.fd_cells, "step": "1e-3 relative, f32 forward",
+                        "relative_error": fd_rel}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="cfg2", choices=["cfg2", "cfg2_spml", "cfg3"])
+    ap.add_argument("--config", default="cfg2", choices=["cfg2", "cfg2_spml", "cfg3", "cfg5_gradient"])
+    ap.add_argument("--fd-cells", type=int, default=4,
+                    help="cfg5_gradient: cells per parameter of the finite-difference spot check")
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="after the run, copy a buffer of this many GiB 3 times (a known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU domain decomposition is "
-                                   "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "a design, not built", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.config == "cfg5_gradient":
+        return gradient_workload(args)
 
     import torch
 
@@ -236,6 +318,15 @@ def main():
         "kernels_ms": {k: round(v * 1e3, 4) for k, v in kt.items()},
         "bytes_per_step": step_bytes, "N_u": n_u, "S_u": s_u,
     }
+    if args.copy_calibration > 0:
+        n = int(args.copy_calibration * (1 << 30)) // 4
+        src_buf = torch.zeros(n, dtype=torch.float32, device="cuda")
+        dst_buf = torch.empty_like(src_buf)
+        for _ in range(3):
+            dst_buf.copy_(src_buf)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = n * 4
+        del src_buf, dst_buf
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

@@ -206,12 +206,14 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
  * and spans (ex, ey, ez) cells.  Its SW_BAND_DEPTH outermost cells along
  * each face form the band; a column is one lateral position of one face
  * (x faces span the full lateral box, y faces exclude the x bands, z faces
- * the x and y bands).  Per column: 72 features (9 fields x 4 band cells
- * after the current step, then the same after the previous step) and 36
- * outputs (corrections of the current band).  Each of the 6 faces has its own
- * network of sw_mlp_param_count() floats:
+ * the x and y bands).  Per column: 72 features x (the 9 fields at the 4
+ * band cells after the current step, cur, then cur minus the corrected band
+ * of the previous step) and 36 outputs (corrections added to cur).  Each of
+ * the 6 faces has its own network of sw_mlp_param_count() floats:
  *   W1[72][64] b1[64] W2[64][64] b2[64] W3[64][36] b3[36] in_scale[72] out_scale[36]
- *   y = (W3^T tanh(W2^T tanh(W1^T (x/in_scale) + b1) + b2) + b3) * out_scale */
+ *   u = x/in_scale, s = RMS of the 72 entries of u,
+ *   y = (W3^T tanh(W2^T tanh(W1^T (u/s) + b1) + b2) + b3) * out_scale * s  (0 if s = 0)
+ * sw_mlp_forward / sw_mlp_loss_backward take samples already divided by s. */
 #define SW_BAND_DEPTH 4
 #define SW_BAND_FEATURES 72
 #define SW_BAND_OUTPUTS 36

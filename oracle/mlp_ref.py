@@ -29,9 +29,25 @@ def forward(p, X):
     return H1, H2, Y
 
 
+def band_features(cur, prev):
+    """Features of band columns: [cur, cur - prev] (72 per column)."""
+    cur = np.asarray(cur, np.float64)
+    return np.concatenate([cur, cur - np.asarray(prev, np.float64)], 1)
+
+
+def column_scale(p, X):
+    """RMS over the 72 channel-normalised features of each column."""
+    u = np.asarray(X, np.float64) / unpack(p)["in_scale"]
+    return np.sqrt(np.mean(u * u, 1))
+
+
 def correction(p, X):
-    """The fused kernel's output: y * out_scale."""
-    return forward(p, X)[2] * unpack(p)["out_scale"]
+    """The fused kernel's correction: s * out_scale * net(X / s), 0 where s = 0."""
+    s = column_scale(p, X)
+    safe = np.where(s > 0, s, 1.0)
+    y = forward(p, np.asarray(X, np.float64) / safe[:, None])[2] * unpack(p)["out_scale"] * safe[:, None]
+    y[s == 0] = 0.0
+    return y
 
 
 def loss_and_grad(p, X, T):

@@ -10,13 +10,16 @@
 // so every band cell belongs to exactly one column.
 //
 // Features of a column: the 9 fields (vx vy vz sxx syy szz sxy sxz syz) at
-// the D band cells after the current step, then the same 36 values after the
-// previous step (as corrected), i.e. 72 = 9 x 4 x 2; band cell d = 0 touches
-// the CPML.  Output: 36 corrections (same order as the first half).
+// the D band cells after the current step (cur), then cur minus the same 36
+// values after the previous step (as corrected), i.e. 72 = 9 x 4 x 2; band
+// cell d = 0 touches the CPML.  Output: 36 corrections added to cur.
 //
 // Network of one face (params, row-major):
 //   W1[72][64] b1[64] W2[64][64] b2[64] W3[64][36] b3[36] in_scale[72] out_scale[36]
-//   y = (W3^T tanh(W2^T tanh(W1^T (x / in_scale) + b1) + b2) + b3) * out_scale
+//   u = x / in_scale,  s = RMS(u) (over the 72 entries),
+//   y = (W3^T tanh(W2^T tanh(W1^T (u / s) + b1) + b2) + b3) * out_scale * s
+// (y = 0 where s = 0).  The training entry points see samples already divided
+// by s: their forward pass is the bracket applied to x / in_scale.
 #include "sw_internal.h"
 
 namespace sw {
@@ -123,7 +126,7 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
     const int a = r % na, b = r / na;
     const float *W1 = w, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH, *B3 = W3 + kH * kOut;
     const float *IS = B3 + kOut, *OS = IS + kIn;
-    float x[kIn];
+    float cur[kOut], x[kIn];
     long pc[kD];
 #pragma unroll
     for (int d = 0; d < kD; ++d) {
@@ -131,44 +134,66 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
         band_cell(q, f, a, b, d, i, j, k);
         pc[d] = ((long)k * ny + j) * nx + i;
 #pragma unroll
-        for (int t = 0; t < kF; ++t) x[d * kF + t] = F.f[t][pc[d]];
+        for (int t = 0; t < kF; ++t) cur[d * kF + t] = F.f[t][pc[d]];
     }
+    // features: the band after the step and its change over the step
 #pragma unroll
-    for (int t = 0; t < kOut; ++t) x[kOut + t] = prev[(long)c * kOut + t];
-    float h1[kH];
+    for (int t = 0; t < kOut; ++t) {
+        x[t] = cur[t];
+        x[kOut + t] = cur[t] - prev[(long)c * kOut + t];
+    }
+    // per-column scale: RMS of the channel-normalised features; the network
+    // sees unit-RMS input and its output is scaled back, so the correction is
+    // proportional to the wavefield (the wave equation is linear)
+    float ss = 0.f;
 #pragma unroll
-    for (int o = 0; o < kH; ++o) h1[o] = B1[o];
     for (int in = 0; in < kIn; ++in) {
-        const float xi = x[in] / IS[in];
-#pragma unroll
-        for (int o = 0; o < kH; ++o) h1[o] += xi * W1[in * kH + o];
+        x[in] = x[in] / IS[in];
+        ss += x[in] * x[in];
     }
-#pragma unroll
-    for (int o = 0; o < kH; ++o) h1[o] = tanhf(h1[o]);
-    float h2[kH];
-#pragma unroll
-    for (int o = 0; o < kH; ++o) h2[o] = B2[o];
-    for (int in = 0; in < kH; ++in) {
-        const float hi = h1[in];
-#pragma unroll
-        for (int o = 0; o < kH; ++o) h2[o] += hi * W2[in * kH + o];
-    }
-#pragma unroll
-    for (int o = 0; o < kH; ++o) h2[o] = tanhf(h2[o]);
+    const float rs = sqrtf(ss / (float)kIn);
     float y[kOut];
+    if (rs > 0.f) {
+        const float inv = 1.f / rs;
+        float h1[kH];
 #pragma unroll
-    for (int o = 0; o < kOut; ++o) y[o] = B3[o];
-    for (int in = 0; in < kH; ++in) {
-        const float hi = h2[in];
+        for (int o = 0; o < kH; ++o) h1[o] = B1[o];
+        for (int in = 0; in < kIn; ++in) {
+            const float xi = x[in] * inv;
 #pragma unroll
-        for (int o = 0; o < kOut; ++o) y[o] += hi * W3[in * kOut + o];
+            for (int o = 0; o < kH; ++o) h1[o] += xi * W1[in * kH + o];
+        }
+#pragma unroll
+        for (int o = 0; o < kH; ++o) h1[o] = tanhf(h1[o]);
+        float h2[kH];
+#pragma unroll
+        for (int o = 0; o < kH; ++o) h2[o] = B2[o];
+        for (int in = 0; in < kH; ++in) {
+            const float hi = h1[in];
+#pragma unroll
+            for (int o = 0; o < kH; ++o) h2[o] += hi * W2[in * kH + o];
+        }
+#pragma unroll
+        for (int o = 0; o < kH; ++o) h2[o] = tanhf(h2[o]);
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) y[o] = B3[o];
+        for (int in = 0; in < kH; ++in) {
+            const float hi = h2[in];
+#pragma unroll
+            for (int o = 0; o < kOut; ++o) y[o] += hi * W3[in * kOut + o];
+        }
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) y[o] = y[o] * OS[o] * rs;
+    } else {
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) y[o] = 0.f;
     }
 #pragma unroll
     for (int d = 0; d < kD; ++d)
 #pragma unroll
         for (int t = 0; t < kF; ++t) {
             const int o = d * kF + t;
-            const float v = x[o] + y[o] * OS[o];
+            const float v = cur[o] + y[o];
             F.f[t][pc[d]] = v;
             prev[(long)c * kOut + o] = v;
         }

@@ -3,8 +3,13 @@
 // Reference: src/Elastic3D_SSG_CPML.py:144-309 (update_SSG) and
 // src/Elastic3D_SSG_Tensor.py:166-345 (same maths, other vz term order).
 //
-// Execution: each thread owns one (i, j) column of an x-y tile (x fastest,
-// 64 lanes = one wavefront row) and marches along z over a chunk of planes.
+// Execution: each thread owns one (i, j) column and marches along z over a
+// chunk of planes; a block takes 256 consecutive columns of a box in x-fastest
+// order.  On large grids a sweep is three launches over boxes of full-width
+// rows, instantiated with the CPML of only the axes whose slabs they meet (x;
+// x and y; x, y and z), so the bulk of the grid runs with fewer registers and
+// more waves in flight; on grids where the CPML shell is a large part, one
+// launch with all three axes.
 //  * The z-differentiated fields live in per-thread register windows of 2N
 //    values that slide with k, so each of their planes is read once.
 //  * Every value a plane needs (x and y stencil taps, material, the
@@ -14,12 +19,14 @@
 //    ~35 independent loads in flight instead of waiting on one branch at a
 //    time.  All addresses of an active column are in bounds, so the loads
 //    need no per-component predicates.
-//  * Blocks are renumbered so that the tiles of one neighbourhood run on the
+//  * Blocks are renumbered so that the blocks of one neighbourhood run on the
 //    same XCD (MI355X dispatches consecutive workgroups round-robin over 8
 //    XCDs, each with its own L2) and halo lines are shared within one L2.
 //  * The z slab of the CPML is a wave-uniform condition; x and y slabs are
 //    per lane.
 // Arithmetic is term for term the reference's (build with -ffp-contract=off).
+#include <algorithm>
+
 #include "sw_internal.h"
 
 // SW_PREFETCH=1 double-buffers the register plane (fetch k+1 while computing k).
@@ -40,27 +47,51 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb)
     return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
+// A launch covers up to kMaxBox boxes of cells: columns [i0, i0+w) x
+// [j0, j0+h) over planes [k0, k1).  Each box is cut into blocks of kThreads
+// consecutive columns (x fastest, so no lane idles on a ragged x edge and a
+// wave reads at most two row segments per tap) times chunks of cz planes.
+constexpr int kMaxBox = 2;
+constexpr int kThreads = 256;
+
+struct Box {
+    int i0, j0, w, h, k0, k1, ncb;  // ncb: column blocks of the box
+};
+
 struct Geo {
-    int nbx, nby, nbz, cz;
+    int nbox, cz;
+    Box box[kMaxBox];
+    int blk0[kMaxBox + 1];  // first block of each box; blk0[nbox] = total
 };
 
 struct Cell {
     int i, j, k0, k1;
 };
 
-template <int TX, int TY>
-__device__ __forceinline__ Cell tile_cell(const sw_ssg3d &g, const Geo geo, int N)
+__device__ __forceinline__ Cell box_cell(const Geo &geo)
 {
     const int b = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int bx = b % geo.nbx;
-    const int r = b / geo.nbx;
-    const int by = r % geo.nby;
-    const int bz = r / geo.nby;
+    // wave-uniform selection of the box (no dynamic indexing of the argument)
+    Box B = geo.box[0];
+    int b0 = 0;
+#pragma unroll
+    for (int t = 1; t < kMaxBox; ++t)
+        if (t < geo.nbox && b >= geo.blk0[t]) {
+            B = geo.box[t];
+            b0 = geo.blk0[t];
+        }
+    const int lb = b - b0;
+    const int cb = lb % B.ncb, zb = lb / B.ncb;
+    const int t = cb * kThreads + (int)threadIdx.x;
     Cell c;
-    c.i = bx * TX + threadIdx.x;
-    c.j = by * TY + threadIdx.y;
-    c.k0 = N + bz * geo.cz;
-    c.k1 = min(c.k0 + geo.cz, g.nz - N);
+    if (t >= B.w * B.h) {
+        c.i = -1; c.j = -1;
+    } else {
+        c.i = B.i0 + t % B.w;
+        c.j = B.j0 + t / B.w;
+    }
+    c.k0 = B.k0 + zb * geo.cz;
+    c.k1 = min(c.k0 + geo.cz, B.k1);
     return c;
 }
 
@@ -161,14 +192,15 @@ struct Slab {
     uint32_t nxs, nys, nzs;
 };
 
+template <int PM>
 __device__ __forceinline__ Slab slab_setup(const sw_ssg3d &g, int i, int j)
 {
     Slab s;
     const int wx = g.px.lo + (g.nx - g.px.hi);
     const int wy = g.py.lo + (g.ny - g.py.hi);
     const int wz = g.pz.lo + (g.nz - g.pz.hi);
-    s.xs = slab_index(i, g.px.lo, g.px.hi);
-    s.ys = slab_index(j, g.py.lo, g.py.hi);
+    s.xs = (PM & 1) ? slab_index(i, g.px.lo, g.px.hi) : -1;
+    s.ys = (PM & 2) ? slab_index(j, g.py.lo, g.py.hi) : -1;
     s.rx = 4u * (uint32_t)(j * wx + s.xs);
     s.ry = 4u * (uint32_t)(s.ys * g.nx + i);
     s.sx = 4u * (uint32_t)(g.ny * wx);
@@ -220,7 +252,7 @@ struct VPlane {
 };
 
 // p: byte offset of (i, j, k); pc: of (i, j, 0); SY, SZ: byte strides.
-template <int N>
+template <int N, int PM>
 __device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
                                             const Slab &sl)
 {
@@ -243,7 +275,7 @@ __device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, con
     P.wxz = ldu(R.sxz, vb, c0 + (N - 1) * (int)SZ);
     P.wyz = ldu(R.syz, vb, c0 + (N - 1) * (int)SZ);
     P.wzz = ldu(R.szz, vb, c0 + N * (int)SZ);
-    P.zs = slab_index(k, g.pz.lo, g.pz.hi);
+    P.zs = (PM & 4) ? slab_index(k, g.pz.lo, g.pz.hi) : -1;
     P.cz = coef_of(g.pz, k, P.zs >= 0);
     const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
     const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
@@ -255,10 +287,10 @@ __device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, con
     }
 }
 
-template <int N, int CLS, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY) k_velocity(sw_ssg3d g, Geo geo)
+template <int N, int CLS, int PM>
+__global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
 {
-    const Cell cl = tile_cell<TX, TY>(g, geo, N);
+    const Cell cl = box_cell(geo);
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (i < N || i >= nx - N || j < N || j >= ny - N) return;
@@ -271,7 +303,7 @@ __global__ void __launch_bounds__(TX *TY) k_velocity(sw_ssg3d g, Geo geo)
     const bool vx_col = i < nx - N - 1 && j >= N + 1;
     const bool vy_col = j < ny - N - 1 && i >= N + 1;
     const bool vz_col = i >= N + 1 && j >= N + 1;
-    const Slab sl = slab_setup(g, i, j);
+    const Slab sl = slab_setup<PM>(g, i, j);
     const Res R = make_res(g);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
     const Coef cy = coef_of(g.py, j, sl.ys >= 0);
@@ -287,14 +319,14 @@ __global__ void __launch_bounds__(TX *TY) k_velocity(sw_ssg3d g, Geo geo)
     }
     VPlane<N> P;
 #if SW_PREFETCH
-    load_vplane<N>(P, g, R, pc, k0, SY, SZ, sl);
+    load_vplane<N, PM>(P, g, R, pc, k0, SY, SZ, sl);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         VPlane<N> Q;
-        if (k + 1 < cl.k1) load_vplane<N>(Q, g, R, pc, k + 1, SY, SZ, sl);
+        if (k + 1 < cl.k1) load_vplane<N, PM>(Q, g, R, pc, k + 1, SY, SZ, sl);
 #else
-        load_vplane<N>(P, g, R, pc, k, SY, SZ, sl);
+        load_vplane<N, PM>(P, g, R, pc, k, SY, SZ, sl);
 #endif
         const uint32_t p = pc + (uint32_t)k * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
@@ -391,7 +423,7 @@ struct SPlane {
     int zs;
 };
 
-template <int N>
+template <int N, int PM>
 __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
                                             const Slab &sl)
 {
@@ -418,7 +450,7 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
     P.wvx = ldu(R.vx, vb, c0 + N * (int)SZ);
     P.wvy = ldu(R.vy, vb, c0 + N * (int)SZ);
     P.wvz = ldu(R.vz, vb, c0 + (N - 1) * (int)SZ);
-    P.zs = slab_index(k, g.pz.lo, g.pz.hi);
+    P.zs = (PM & 4) ? slab_index(k, g.pz.lo, g.pz.hi) : -1;
     P.cz = coef_of(g.pz, k, P.zs >= 0);
     const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
     const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
@@ -431,10 +463,10 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
     }
 }
 
-template <int N, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY) k_stress(sw_ssg3d g, Geo geo)
+template <int N, int PM>
+__global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
 {
-    const Cell cl = tile_cell<TX, TY>(g, geo, N);
+    const Cell cl = box_cell(geo);
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (i < N || i >= nx - N || j < N || j >= ny - N) return;
@@ -448,7 +480,7 @@ __global__ void __launch_bounds__(TX *TY) k_stress(sw_ssg3d g, Geo geo)
     const bool sxy_col = i >= N + 1 && j >= N + 1;
     const bool sxz_col = i < nx - N - 1;
     const bool syz_col = j < ny - N - 1;
-    const Slab sl = slab_setup(g, i, j);
+    const Slab sl = slab_setup<PM>(g, i, j);
     const Res R = make_res(g);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
     const Coef cy = coef_of(g.py, j, sl.ys >= 0);
@@ -464,14 +496,14 @@ __global__ void __launch_bounds__(TX *TY) k_stress(sw_ssg3d g, Geo geo)
     }
     SPlane<N> P;
 #if SW_PREFETCH
-    load_splane<N>(P, g, R, pc, k0, SY, SZ, sl);
+    load_splane<N, PM>(P, g, R, pc, k0, SY, SZ, sl);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         SPlane<N> Q;
-        if (k + 1 < cl.k1) load_splane<N>(Q, g, R, pc, k + 1, SY, SZ, sl);
+        if (k + 1 < cl.k1) load_splane<N, PM>(Q, g, R, pc, k + 1, SY, SZ, sl);
 #else
-        load_splane<N>(P, g, R, pc, k, SY, SZ, sl);
+        load_splane<N, PM>(P, g, R, pc, k, SY, SZ, sl);
 #endif
         const uint32_t p = pc + (uint32_t)k * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
@@ -557,58 +589,119 @@ __global__ void __launch_bounds__(TX *TY) k_stress(sw_ssg3d g, Geo geo)
     }
 }
 
-#ifndef SW_TY
-#define SW_TY 4
-#endif
 #ifndef SW_CZ0
 #define SW_CZ0 8
 #endif
-constexpr int kTX = 64, kTY = SW_TY;
 
-Geo make_geo(const sw_ssg3d &g, int N, int cz)
+// Cells a sweep updates: [N, n-N) on every axis, in one launch (PM = 7) or
+// in three launches whose boxes all keep full-width x rows (long coalesced
+// segments):
+//   PM = 1: planes and rows outside the z and y slabs (x slab per lane);
+//   PM = 3: rows inside a y slab, planes outside the z slabs;
+//   PM = 7: planes inside a z slab.
+// The mask PM names the axes whose CPML the kernel instantiates; dropping z
+// and y from the bulk of the grid frees the registers of their memory
+// variables and coefficients, so more waves are in flight there.
+struct Plan {
+    Geo g[3];  // PM = 1, 3, 7
+};
+
+void add_box(Geo &geo, int i0, int i1, int j0, int j1, int k0, int k1)
 {
-    Geo geo;
-    geo.cz = cz;
-    geo.nbx = (g.nx + kTX - 1) / kTX;
-    geo.nby = (g.ny + kTY - 1) / kTY;
-    geo.nbz = (g.nz - 2 * N + cz - 1) / cz;
-    return geo;
+    if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
+    Box &b = geo.box[geo.nbox++];
+    b.i0 = i0; b.w = i1 - i0;
+    b.j0 = j0; b.h = j1 - j0;
+    b.k0 = k0; b.k1 = k1;
+    b.ncb = (b.w * b.h + kThreads - 1) / kThreads;
 }
 
-// z planes per block: enough blocks to fill 256 CUs several times over while
-// keeping the window warm-up (2N-1 planes) a small fraction of the chunk.
-int choose_cz(const sw_ssg3d &g, int N)
+// Chunk length: SW_CZ0 planes, doubled (up to 64) while the launch keeps at
+// least 32 blocks per CU, so that the last round of blocks is a small part of
+// a launch and the window warm-up (2N-1 planes per chunk) a small part of
+// large grids.
+void finish(Geo &geo)
 {
-    const int cols = ((g.nx + kTX - 1) / kTX) * ((g.ny + kTY - 1) / kTY);
-    const int nzi = g.nz - 2 * N;
+    auto blocks = [&](int cz) {
+        long n = 0;
+        for (int t = 0; t < geo.nbox; ++t)
+            n += (long)geo.box[t].ncb * ((geo.box[t].k1 - geo.box[t].k0 + cz - 1) / cz);
+        return n;
+    };
     int cz = SW_CZ0;
-    while (cz < 64 && (long)cols * ((nzi + 2 * cz - 1) / (2 * cz)) >= 4096) cz *= 2;
-    return cz > nzi ? nzi : cz;
+    while (cz < 64 && blocks(2 * cz) >= 32L * 256) cz *= 2;
+    geo.cz = cz;
+    geo.blk0[0] = 0;
+    for (int t = 0; t < geo.nbox; ++t)
+        geo.blk0[t + 1] = geo.blk0[t] + geo.box[t].ncb * ((geo.box[t].k1 - geo.box[t].k0 + cz - 1) / cz);
+}
+
+Plan make_plan(const sw_ssg3d &g, int N)
+{
+    Plan p{};
+    const int x0 = N, x1 = g.nx - N, y0 = N, y1 = g.ny - N, z0 = N, z1 = g.nz - N;
+    const int yl = std::max(y0, g.py.lo), yh = std::min(y1, g.py.hi);
+    const int zl = std::max(z0, g.pz.lo), zh = std::min(z1, g.pz.hi);
+    // Three launches pay off only when the bulk box dominates: each extra
+    // launch adds a ramp-up and a tail (measured on MI355X: a 200^3 grid with
+    // 20-cell CPML, bulk 64 % of the cells, runs faster as one launch; the
+    // 600x600x400 grid, bulk 84 %, 5 % faster as three).
+    const double bulk = (yl < yh && zl < zh)
+                            ? (double)(x1 - x0) * (yh - yl) * (zh - zl) / ((double)(x1 - x0) * (y1 - y0) * (z1 - z0))
+                            : 0.0;
+    if (bulk >= 0.75) {
+        add_box(p.g[0], x0, x1, yl, yh, zl, zh);
+        add_box(p.g[1], x0, x1, y0, yl, zl, zh);
+        add_box(p.g[1], x0, x1, yh, y1, zl, zh);
+        add_box(p.g[2], x0, x1, y0, y1, z0, zl);
+        add_box(p.g[2], x0, x1, y0, y1, zh, z1);
+    } else {
+        add_box(p.g[2], x0, x1, y0, y1, z0, z1);
+    }
+    for (Geo &geo : p.g) finish(geo);
+    return p;
+}
+
+template <int N, int CLS, int PM>
+void launch_velocity(const sw_ssg3d &g, const Geo &geo, hipStream_t st)
+{
+    if (geo.nbox)
+        hipLaunchKernelGGL((k_velocity<N, CLS, PM>), dim3(geo.blk0[geo.nbox]), dim3(kThreads), 0, st, g, geo);
+}
+
+template <int N, int PM>
+void launch_stress(const sw_ssg3d &g, const Geo &geo, hipStream_t st)
+{
+    if (geo.nbox)
+        hipLaunchKernelGGL((k_stress<N, PM>), dim3(geo.blk0[geo.nbox]), dim3(kThreads), 0, st, g, geo);
 }
 
 }  // namespace
 
 int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st)
 {
-    const dim3 blk(kTX, kTY, 1);
     SW_DISPATCH_ORDER(g->order, {
-        const Geo geo = make_geo(*g, N, choose_cz(*g, N));
-        const dim3 grd(geo.nbx * geo.nby * geo.nbz);
-        if (g->cls == SW_CLASS_TENSOR)
-            hipLaunchKernelGGL((k_velocity<N, SW_CLASS_TENSOR, kTX, kTY>), grd, blk, 0, st, *g, geo);
-        else
-            hipLaunchKernelGGL((k_velocity<N, SW_CLASS_CPML, kTX, kTY>), grd, blk, 0, st, *g, geo);
+        const Plan p = make_plan(*g, N);
+        if (g->cls == SW_CLASS_TENSOR) {
+            launch_velocity<N, SW_CLASS_TENSOR, 1>(*g, p.g[0], st);
+            launch_velocity<N, SW_CLASS_TENSOR, 3>(*g, p.g[1], st);
+            launch_velocity<N, SW_CLASS_TENSOR, 7>(*g, p.g[2], st);
+        } else {
+            launch_velocity<N, SW_CLASS_CPML, 1>(*g, p.g[0], st);
+            launch_velocity<N, SW_CLASS_CPML, 3>(*g, p.g[1], st);
+            launch_velocity<N, SW_CLASS_CPML, 7>(*g, p.g[2], st);
+        }
     });
     return launch_status("ssg3d velocity");
 }
 
 int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st)
 {
-    const dim3 blk(kTX, kTY, 1);
     SW_DISPATCH_ORDER(g->order, {
-        const Geo geo = make_geo(*g, N, choose_cz(*g, N));
-        const dim3 grd(geo.nbx * geo.nby * geo.nbz);
-        hipLaunchKernelGGL((k_stress<N, kTX, kTY>), grd, blk, 0, st, *g, geo);
+        const Plan p = make_plan(*g, N);
+        launch_stress<N, 1>(*g, p.g[0], st);
+        launch_stress<N, 3>(*g, p.g[1], st);
+        launch_stress<N, 7>(*g, p.g[2], st);
     });
     return launch_status("ssg3d stress");
 }

@@ -9,14 +9,22 @@ samples generated by the plain solver itself.
 Training targets (teacher forcing).  A wide-CPML run of a random model is the
 reference.  A thin-CPML run of the same interior is reset to the reference's
 interior state before every step; after the step, the features are the thin
-run's band (9 fields x 4 cells) and the band before the step, and the target
-is the reference band minus the thin band: the one-step error the thin
-boundary makes next to itself.  At inference the corrected band is fed back as
-the "previous" half of the features, as in training.
+run's band (9 fields x 4 cells, "cur") and its change over the step (cur minus
+the band before the step), and the target is the reference band minus cur:
+the one-step error the thin boundary makes next to itself.  At inference the
+corrected band of the previous step takes the place of "the band before the
+step", as in training (with a perfect correction the two coincide).
 
-Configuration: configs' 9 field components imply 3-D, so the training runs are
-small 3-D boxes (layered, perturbed models, random moment-tensor sources) and
-the evaluation is the 200^3 homogeneous case against a 20-cell CPML.
+Scale.  The wave equation is linear, so the correction is made positively
+homogeneous: each column's features are divided by their RMS s (after a fixed
+per-channel scaling) before the network and its output is multiplied by s.
+Training samples are drawn with probability proportional to s^2, so the loss
+is the raw squared one-step error.
+
+The band carries all 9 field components of the 3-D solver, so the training
+runs are small 3-D boxes (layered, perturbed models, random moment-tensor
+sources) and the evaluation is a 200^3 homogeneous case against a 20-cell
+CPML.
 """
 from __future__ import annotations
 
@@ -108,9 +116,10 @@ class BandNet:
 
 
 # ------------------------------------------------------------ data generation
-def random_layered_model(shape, rng, vp_range=(2000.0, 4500.0), nlayers=4, pert=0.05):
+def random_layered_model(shape, rng, vp_range=(1800.0, 4000.0), nlayers=4, pert=0.05):
     """Random layered model along the last axis with a smooth random
-    perturbation; vs = vp/1.73, Gardner density."""
+    perturbation (clipped to +-2 standard deviations); vs = vp/1.73, Gardner
+    density."""
     from scipy import ndimage
     nz = shape[-1]
     cuts = np.sort(rng.integers(1, nz - 1, nlayers - 1))
@@ -122,14 +131,23 @@ def random_layered_model(shape, rng, vp_range=(2000.0, 4500.0), nlayers=4, pert=
     vp = np.broadcast_to(prof, shape).astype(np.float64)
     noise = ndimage.gaussian_filter(rng.standard_normal(shape), 3.0, mode="wrap")
     noise /= noise.std() + 1e-12
-    vp = vp * (1 + pert * noise)
+    vp = vp * (1 + pert * np.clip(noise, -2.0, 2.0))
     vs = vp / 1.73
     rho = 1741.0 * (vp / 1000.0) ** 0.25
     return vs.astype(np.float32), vp.astype(np.float32), rho.astype(np.float32)
 
 
+def courant(vp_max, d, dt, order=2):
+    """vp_max dt / d * sqrt(3) * sum|c|: the 3-D staggered scheme is stable below 1."""
+    from .coeffs import diff_coff
+    return float(vp_max) * dt / d * np.sqrt(3.0) * float(np.sum(np.abs(diff_coff(order))))
+
+
 def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
     from .Elastic3D_SSG_Tensor import ElasticWAVE
+    cr = courant(vp.max(), d, dt)
+    if cr >= 0.95:
+        raise ValueError(f"model violates the stability limit (Courant number {cr:.3f})")
     nt = len(amps)
     mt3 = np.zeros((3, 3), np.float32)
     w = ElasticWAVE(vs, vp, rho, d, d, d, dt, *cell, np.zeros((1, 1), np.int64) + cell[0],
@@ -147,7 +165,14 @@ def grid_size(n_int, npml, s=2):
 
 
 def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20, keep=0.2):
-    """One teacher-forced run: returns per-face (X [m,72], T [m,36]) sample lists."""
+    """One teacher-forced run.
+
+    A wide-CPML run of a random model is the reference; a thin-CPML run of the
+    same interior is reset to the reference interior before every step.  For
+    every band column: features [cur, cur - prev] (cur = thin band after the
+    step, prev = the band before it) and target = reference band - cur.
+    Returns per-face (X [m, 72], T [m, 36]) on the device, a random ``keep``
+    fraction of the columns of every step after the first 10."""
     lib = _sig()
     nw, nt_ = grid_size(n_int, wide), grid_size(n_int, thin)
     off = wide - thin
@@ -175,73 +200,160 @@ def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=15.0, thin=4, wide=
     # interior boxes in device layout [z][y][x]
     ti = (slice(bt.oz, bt.oz + bt.ez), slice(bt.oy, bt.oy + bt.ey), slice(bt.ox, bt.ox + bt.ex))
     wi = (slice(bw.oz, bw.oz + bw.ez), slice(bw.oy, bw.oy + bw.ey), slice(bw.ox, bw.ox + bw.ex))
+    gen = torch.Generator(device=dev).manual_seed(int(rng.integers(1 << 31)))
     for n in range(steps):
         for nm in names:
             T_._e.f[nm][ti].copy_(W._e.f[nm][wi])
-        gt, gw = T_._e.descriptor(), W._e.descriptor()
-        L.check(lib.sw_band_gather(gt, C.byref(bt), L.fptr(prv), st), "band gather")
+        L.check(lib.sw_band_gather(T_._e.descriptor(), C.byref(bt), L.fptr(prv), st), "band gather")
         T_.run(1, n)
         W.run(1, n)
-        gt, gw = T_._e.descriptor(), W._e.descriptor()
-        L.check(lib.sw_band_gather(gt, C.byref(bt), L.fptr(cur), st), "band gather")
-        L.check(lib.sw_band_gather(gw, C.byref(bw), L.fptr(wcur), st), "band gather")
+        L.check(lib.sw_band_gather(T_._e.descriptor(), C.byref(bt), L.fptr(cur), st), "band gather")
+        L.check(lib.sw_band_gather(W._e.descriptor(), C.byref(bw), L.fptr(wcur), st), "band gather")
         if n < 10:
             continue
-        x = torch.cat([cur, prv], 1)
+        x = torch.cat([cur, cur - prv], 1)
         t = wcur - cur
         for f_ in range(6):
             a, b = int(col0[f_]), int(col0[f_ + 1])
             k = max(1, int(keep * (b - a)))
-            idx = torch.randint(a, b, (k,), device=dev)
+            idx = torch.randint(a, b, (k,), device=dev, generator=gen)
             xs[f_].append(x[idx])
             ts[f_].append(t[idx])
     return {f_: (torch.cat(xs[f_]), torch.cat(ts[f_])) for f_ in range(6)}
 
 
-def train_face(net: BandNet, face, X, T, iters=3000, batch=2048, lr=2e-3, seed=0, log=None):
-    """Fit one face network with the HIP training kernels; returns the loss history."""
-    lib = _sig()
+def normalise(X, T, in_scale):
+    """Divide every sample by its column scale s = RMS(X / in_scale); drop
+    silent samples.  Returns (X / s, T / s, s)."""
+    s = torch.sqrt(torch.mean((X / in_scale) ** 2, 1))
+    keep = s > 0
+    X, T, s = X[keep], T[keep], s[keep]
+    return X / s[:, None], T / s[:, None], s
+
+
+class _Trainer:
+    """HIP forward / backward / Adam of one face network (buffers for one batch size)."""
+
+    def __init__(self, net: BandNet, face, batch):
+        self.lib, self.net, self.face, self.batch = _sig(), net, face, batch
+        dev = net.device
+        self.ntrain = net.lay["b3"][1]  # weights and biases come first
+        self.grad = torch.zeros(self.ntrain, device=dev)
+        self.m1 = torch.zeros(self.ntrain, device=dev)
+        self.m2 = torch.zeros(self.ntrain, device=dev)
+        self.H1 = torch.empty((batch, NH), device=dev)
+        self.H2 = torch.empty((batch, NH), device=dev)
+        self.Y = torch.empty((batch, NOUT), device=dev)
+        self.work = torch.empty(batch * (NOUT + 2 * NH), device=dev)
+        self.rows = torch.empty(batch, device=dev)
+        self.st = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        self.t = 0
+
+    def step(self, xb, tb, lr):
+        p = self.net.params[self.face]
+        lib, B = self.lib, xb.shape[0]
+        L.check(lib.sw_mlp_forward(L.fptr(p), L.fptr(xb), B, L.fptr(self.H1), L.fptr(self.H2), L.fptr(self.Y),
+                                   self.st), "mlp forward")
+        L.check(lib.sw_mlp_loss_backward(L.fptr(p), L.fptr(xb), L.fptr(self.H1), L.fptr(self.H2), L.fptr(self.Y),
+                                         L.fptr(tb), B, L.fptr(self.grad), L.fptr(self.work), L.fptr(self.rows),
+                                         self.st), "mlp backward")
+        self.t += 1
+        L.check(lib.sw_adam(L.fptr(p), L.fptr(self.grad), L.fptr(self.m1), L.fptr(self.m2), self.ntrain, lr, 0.9,
+                            0.999, 1e-8, self.t, self.st), "adam")
+
+    def predict(self, Xn):
+        """Network output (in units of out_scale) for normalised samples, in batches."""
+        p = self.net.params[self.face]
+        out = torch.empty((Xn.shape[0], NOUT), device=Xn.device)
+        for a in range(0, Xn.shape[0], self.batch):
+            xb = Xn[a:a + self.batch].contiguous()
+            B = xb.shape[0]
+            L.check(self.lib.sw_mlp_forward(L.fptr(p), L.fptr(xb), B, L.fptr(self.H1), L.fptr(self.H2),
+                                            L.fptr(self.Y), self.st), "mlp forward")
+            out[a:a + B] = self.Y[:B]
+        return out
+
+
+def residual_fraction(trainer, X, T, in_scale, out_scale):
+    """sum |T - correction(X)|^2 / sum |T|^2 over raw samples: the one-step
+    error left after the correction, relative to the error without it."""
+    s = torch.sqrt(torch.mean((X / in_scale) ** 2, 1))
+    live = s > 0
+    corr = torch.zeros_like(T)
+    corr[live] = trainer.predict((X[live] / s[live, None]).contiguous()) * out_scale * s[live, None]
+    num = torch.sum((T.double() - corr.double()) ** 2)
+    return float(num / torch.sum(T.double() ** 2).clamp_min(1e-300))
+
+
+def calibrate(trainer, X, T, in_scale, out_scale):
+    """Least-squares output scale alpha = <T, C> / <C, C> of the correction C
+    on (X, T), clamped to [0, 1]: for 0 < alpha <= 2<T,C>/<C,C> the one-step
+    error |T - alpha C|^2 is below |T|^2 whenever <T, C> > 0."""
+    s = torch.sqrt(torch.mean((X / in_scale) ** 2, 1))
+    live = s > 0
+    corr = trainer.predict((X[live] / s[live, None]).contiguous()).double() * out_scale.double() * s[live, None]
+    tc = float(torch.sum(T[live].double() * corr))
+    cc = float(torch.sum(corr * corr))
+    return min(max(tc / cc, 0.0), 1.0) if cc > 0 else 0.0
+
+
+def train_face(net: BandNet, face, X, T, Xv=None, Tv=None, iters=3000, batch=4096, lr=2e-3, seed=0, log=None,
+               eval_every=250):
+    """Fit one face network with the HIP training kernels.
+
+    Samples are drawn with probability proportional to s^2 (their squared
+    column scale), so the mean loss over a batch of normalised samples
+    estimates the raw squared error.  With a validation set (Xv, Tv) the
+    parameters with the lowest validation residual are kept, then the output
+    scale is calibrated on the validation set (``calibrate``).  Returns the
+    history [(iter, train residual fraction, validation residual fraction)];
+    its last entry is after calibration."""
     dev = net.device
-    # normalisation: RMS per input channel / per output channel
-    in_scale = torch.sqrt(torch.mean(X * X, 0)) + 1e-20
-    out_scale = torch.sqrt(torch.mean(T * T, 0)) + 1e-20
+    in_scale = torch.sqrt(torch.mean(X * X, 0)) + 1e-30
+    out_scale = torch.sqrt(torch.mean(T * T, 0)) + 1e-30
     net.set_scales(face, in_scale, out_scale)
-    p = net.params[face]
-    ntrain = net.lay["b3"][1]  # weights and biases come first
-    grad = torch.zeros(ntrain, device=dev)
-    m1 = torch.zeros(ntrain, device=dev)
-    m2 = torch.zeros(ntrain, device=dev)
-    H1 = torch.empty((batch, NH), device=dev)
-    H2 = torch.empty((batch, NH), device=dev)
-    Y = torch.empty((batch, NOUT), device=dev)
-    work = torch.empty(batch * (NOUT + 2 * NH), device=dev)
-    loss_rows = torch.empty(batch, device=dev)
+    Xn, Tn, s = normalise(X, T, in_scale)
+    w = (s.double() ** 2)
+    w = w / w.sum()
+    tr = _Trainer(net, face, batch)
     g = torch.Generator(device=dev).manual_seed(seed)
-    st = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-    hist = []
-    n = X.shape[0]
+    n_eval = min(X.shape[0], 200000)
+
+    def evaluate(os_=out_scale):
+        a = residual_fraction(tr, X[:n_eval], T[:n_eval], in_scale, os_)
+        b = residual_fraction(tr, Xv, Tv, in_scale, os_) if Xv is not None else float("nan")
+        return a, b
+
+    hist = [(0,) + evaluate()]
+    best = (hist[0][2] if Xv is not None else hist[0][1], net.params[face].clone())
     for it in range(1, iters + 1):
-        idx = torch.randint(0, n, (batch,), device=dev, generator=g)
-        xb = X[idx].contiguous()
-        tb = T[idx].contiguous()
-        L.check(lib.sw_mlp_forward(L.fptr(p), L.fptr(xb), batch, L.fptr(H1), L.fptr(H2), L.fptr(Y), st), "mlp fwd")
-        L.check(lib.sw_mlp_loss_backward(L.fptr(p), L.fptr(xb), L.fptr(H1), L.fptr(H2), L.fptr(Y), L.fptr(tb),
-                                         batch, L.fptr(grad), L.fptr(work), L.fptr(loss_rows), st), "mlp bwd")
-        cur_lr = lr * (0.1 ** (it / iters))
-        L.check(lib.sw_adam(L.fptr(p), L.fptr(grad), L.fptr(m1), L.fptr(m2), ntrain, cur_lr, 0.9, 0.999, 1e-8,
-                            it, st), "adam")
-        if it % 100 == 0 or it == 1:
-            loss = float(loss_rows.sum().item()) / (batch * NOUT)
-            hist.append((it, loss))
+        idx = torch.multinomial(w, batch, replacement=True, generator=g)
+        tr.step(Xn[idx].contiguous(), Tn[idx].contiguous(), lr * (0.1 ** (it / iters)))
+        if it % eval_every == 0 or it == iters:
+            hist.append((it,) + evaluate())
+            score = hist[-1][2] if Xv is not None else hist[-1][1]
+            if score < best[0]:
+                best = (score, net.params[face].clone())
             if log:
-                log(f"face {FACES[face]} iter {it} loss {loss:.4e}")
+                log(f"face {FACES[face]} iter {it} train {hist[-1][1]:.4f} val {hist[-1][2]:.4f}")
+    net.params[face].copy_(best[1])
+    # calibrate the output scale on held-out samples (the training ones without)
+    cx, ct = (Xv, Tv) if Xv is not None else (X, T)
+    alpha = calibrate(tr, cx, ct, in_scale, out_scale)
+    net.set_scales(face, in_scale, out_scale * alpha)
+    hist.append((iters + 1,) + evaluate(out_scale * alpha))
+    if log:
+        log(f"face {FACES[face]} output scale {alpha:.3f}: train {hist[-1][1]:.4f} val {hist[-1][2]:.4f}")
     return hist
 
 
-def train(n_runs=16, n_int=40, steps=160, iters=3000, seed=0, log=print):
-    """Generate teacher-forced samples with the plain solver and train all faces."""
+def train(n_runs=16, n_int=40, steps=160, iters=3000, seed=0, log=print, n_val=None, batch=4096, lr=2e-3):
+    """Generate teacher-forced samples with the plain solver and train all six
+    face networks; the last ``n_val`` runs (default n_runs // 8, at least 1)
+    are held out for validation."""
     rng = np.random.default_rng(seed)
     t0 = time.time()
+    n_val = max(1, n_runs // 8) if n_val is None else n_val
     pools = {f: ([], []) for f in range(6)}
     for r in range(n_runs):
         data = collect_run(rng, n_int=n_int, steps=steps)
@@ -252,10 +364,13 @@ def train(n_runs=16, n_int=40, steps=160, iters=3000, seed=0, log=print):
             log(f"run {r + 1}/{n_runs} collected ({time.time() - t0:.1f} s)")
     net = BandNet(seed=seed)
     hists = {}
+    nt = n_runs - n_val
     for f in range(6):
-        X = torch.cat(pools[f][0])
-        T = torch.cat(pools[f][1])
-        hists[f] = train_face(net, f, X, T, iters=iters, seed=seed + f, log=log)
+        X = torch.cat(pools[f][0][:nt])
+        T = torch.cat(pools[f][1][:nt])
+        Xv = torch.cat(pools[f][0][nt:]) if n_val else None
+        Tv = torch.cat(pools[f][1][nt:]) if n_val else None
+        hists[f] = train_face(net, f, X, T, Xv, Tv, iters=iters, batch=batch, lr=lr, seed=seed + f, log=log)
     return net, hists
 
 
@@ -283,6 +398,22 @@ class LearnedRun:
 
 
 # --------------------------------------------------------------- evaluation
+def tune_output_scale(net: BandNet, alphas=(1.0, 0.5, 0.25, 0.1, 0.05), n_int=28, steps=150, seed=7):
+    """Closed-loop calibration: scale every face's output by the alpha in
+    ``alphas`` (or 0) that gives the least reflected energy on one held-out
+    model, and keep that scale.  Returns (alpha, {alpha: energy})."""
+    a, b = net.lay["out_scale"]
+    base = net.params[:, a:b].clone()
+    energy = {}
+    for al in alphas:
+        net.params[:, a:b] = base * al
+        energy[al] = reflection_study(net, n_int=n_int, steps=steps, seed=seed)["learned"]
+    energy[0.0] = reflection_study(net, n_int=n_int, steps=steps, seed=seed)["thin"]
+    best = min(energy, key=energy.get)
+    net.params[:, a:b] = base * best
+    return best, energy
+
+
 def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20,
                      model=None, src=None, every=5, timing_steps=0):
     """Reflected energy of thin CPML (with and without the network) and of the

@@ -66,3 +66,38 @@ def cfl_dt(vp_max: float, d, courant: float = 0.45) -> float:
     number, e.g. test3d_homegenous.py:35)."""
     d = np.atleast_1d(np.asarray(d, dtype=np.float64))
     return float(courant / (vp_max * np.sqrt(np.sum(1.0 / d ** 2))))
+
+
+def layered_2d_with_anomalies(shape, d: float, seed: int = 0, n_layers: int = 6, vp_top: float = 2000.0,
+                              vp_bottom: float = 4500.0, amplitude: float = 0.10, radius: float = 150.0):
+    """2-D (nx, nz) layered background and the same with two Gaussian
+    anomalies, +amplitude and -amplitude in vp and vs, of e-folding radius
+    ``radius`` metres, at seeded positions in the middle third of the depth.
+    Interfaces are gently dipping, layer vp rises linearly with depth,
+    vs = vp / 1.73, Gardner density (unchanged by the anomalies).
+    Returns ((vp0, vs0, rho), (vp1, vs1, rho)) as float32 (nx, nz) arrays."""
+    rng = np.random.default_rng(seed)
+    nx, nz = shape
+    x = np.arange(nx, dtype=np.float64) * d
+    z = np.arange(nz, dtype=np.float64) * d
+    depth = nz * d
+    zbar = np.linspace(0.12, 0.88, n_layers - 1) * depth
+    layer = np.zeros(shape, dtype=np.int16)
+    for m in range(n_layers - 1):
+        dip = rng.uniform(-0.05, 0.05)
+        surf = zbar[m] + dip * (x - x.mean())
+        layer += (z[None, :] >= surf[:, None]).astype(np.int16)
+    vlayers = np.linspace(vp_top, vp_bottom, n_layers)
+    vp0 = vlayers[layer]
+    vs0 = vp0 / 1.73
+    rho = 1741.0 * (vp0 / 1000.0) ** 0.25
+    bump = np.zeros(shape)
+    for sign in (1.0, -1.0):
+        cx = rng.uniform(0.25, 0.75) * nx * d
+        cz = rng.uniform(1.0 / 3.0, 2.0 / 3.0) * depth
+        r2 = (x[:, None] - cx) ** 2 + (z[None, :] - cz) ** 2
+        bump += sign * np.exp(-r2 / radius ** 2)
+    vp1 = vp0 * (1.0 + amplitude * bump)
+    vs1 = vs0 * (1.0 + amplitude * bump)
+    f = lambda a: a.astype(np.float32)  # noqa: E731
+    return (f(vp0), f(vs0), f(rho)), (f(vp1), f(vs1), f(rho))

@@ -37,3 +37,18 @@ def test_perturbation_statistics():
 def test_cfl_dt():
     dt = cfl_dt(5500.0, (12.5, 12.5, 12.5), 0.45)
     assert abs(5500.0 * dt * np.sqrt(3 / 12.5 ** 2) - 0.45) < 1e-12
+
+
+def test_2d_anomaly_model_is_seeded_and_bounded():
+    from seiswave_taichi_amd.models import layered_2d_with_anomalies
+    (vp0, vs0, rho), (vp1, vs1, rho1) = layered_2d_with_anomalies((200, 90), 10.0, seed=3)
+    (wp0, _, _), (wp1, _, _) = layered_2d_with_anomalies((200, 90), 10.0, seed=3)
+    np.testing.assert_array_equal(vp1, wp1)
+    np.testing.assert_array_equal(vp0, wp0)
+    r = vp1 / vp0
+    assert 1.095 < r.max() <= 1.1 + 1e-6 and 0.9 - 1e-6 <= r.min() < 0.905
+    np.testing.assert_allclose(vs1 / vs0, r, rtol=1e-6)
+    np.testing.assert_allclose(vp0 / vs0, 1.73, rtol=1e-6)
+    np.testing.assert_array_equal(rho, rho1)
+    # background: vp non-decreasing with depth in every column
+    assert np.all(np.diff(vp0, axis=1) >= 0)
