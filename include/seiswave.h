@@ -198,10 +198,22 @@ int sw_ssg2d_adjoint_inject(const sw_ssg2d *g, const sw_adj2d *adj, const sw_rec
 /* gvp, gvs, grho [nz][nx] from the accumulated coefficient gradients */
 int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float *vp, const float *vs,
                             float *gvp, float *gvs, float *grho, void *stream);
+/* One checkpoint segment of the adjoint, steps it0 .. it0+n-1, in one call.
+ * g holds the forward state at step it0 as one block of 15 planes of nx*nz
+ * floats starting at g->vx (vx vz sxx szz sxz, then the 10 memory planes at
+ * g->mem); the call advances it to step it0+n (source injection, velocity
+ * and stress sweeps of src/Elastic2D_CPML.py:127-200 per step), keeping the
+ * state before and after the velocity sweep of every step in work (2*n*15*
+ * nx*nz floats, caller-owned).  Then, for it = it0+n-1 down to it0, it makes
+ * the same calls as sw_ssg2d_adjoint_inject(residual, it),
+ * sw_ssg2d_adjoint_stress (state after the velocity sweep) and
+ * sw_ssg2d_adjoint_velocity (state before it).  src: explosive or moment
+ * tensor (or none). */
+int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_adj2d *adj,
+                             const sw_receivers *residual, float *work, int32_t it0, int32_t n, void *stream);
 
-/* ---- learned thin boundary (no counterpart in the reference: the
- * north-star replacement of most of a wide CPML by a thin CPML plus a
- * per-boundary-column MLP) ----
+/* ---- learned thin boundary (no counterpart in the reference: a thin CPML
+ * plus a per-boundary-column MLP in place of most of a wide CPML) ----
  * The interior box (cells outside every CPML slab) starts at (ox, oy, oz)
  * and spans (ex, ey, ez) cells.  Its SW_BAND_DEPTH outermost cells along
  * each face form the band; a column is one lateral position of one face

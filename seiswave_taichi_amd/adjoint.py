@@ -10,7 +10,8 @@ of the seismograms with respect to vp, vs and rho.
 Memory: the forward keeps a checkpoint of the full state every
 ``checkpoint`` steps; the backward recomputes each segment forward from its
 checkpoint, keeping the state before and after the velocity stage of every
-step of the segment, then runs the adjoint backwards through it.
+step of the segment, then runs the adjoint backwards through it, one native
+call per segment (sw_ssg2d_adjoint_segment).
 
 Sources inject into the stresses (explosive or moment tensor), which do not
 depend on the model; force sources are refused because their 1/rho factor is
@@ -181,14 +182,10 @@ class Elastic2DModeling:
         acc = torch.zeros((5, nz, nx), dtype=torch.float32, device=self.device)  # g_b1 g_b4 g_lamh g_muh g_muz
         adj = self.new_state()
         tmp = torch.zeros((4, nz, nx), dtype=torch.float32, device=self.device)
-        pre = self.new_state(K)
-        mid = self.new_state(K)
+        seg = self.new_state(2 * K)  # states before / after the velocity sweep of each step of a segment
         work = self.new_state()
         q = self._adj_desc(adj, tmp, acc)
         gw = self.desc(work)
-        gpre = [self.desc(pre[t]) for t in range(K)]
-        gmid = [self.desc(mid[t]) for t in range(K)]
-        plane = nx * nz
         for s, shot in enumerate(self.shots):
             res = grad_data[s].detach().to(torch.float32).contiguous()
             rres = self.receivers(res)
@@ -197,23 +194,12 @@ class Elastic2DModeling:
             for s0 in reversed(range(0, self.nt, K)):
                 n = min(K, self.nt - s0)
                 work.copy_(checkpoints[s][s0 // K])
-                for t in range(n):
-                    it = s0 + t
-                    L.check(lib.sw_ssg2d_inject(gw, src, it, st), "sw_ssg2d_inject")
-                    pre[t].copy_(work)
-                    L.check(lib.sw_ssg2d_velocity(gw, st), "sw_ssg2d_velocity")
-                    mid[t].copy_(work)
-                    L.check(lib.sw_ssg2d_stress(gw, st), "sw_ssg2d_stress")
-                for t in reversed(range(n)):
-                    it = s0 + t
-                    L.check(lib.sw_ssg2d_adjoint_inject(gw, q, rres, it, st), "sw_ssg2d_adjoint_inject")
-                    L.check(lib.sw_ssg2d_adjoint_stress(gmid[t], q, st), "sw_ssg2d_adjoint_stress")
-                    L.check(lib.sw_ssg2d_adjoint_velocity(gpre[t], q, st), "sw_ssg2d_adjoint_velocity")
+                L.check(lib.sw_ssg2d_adjoint_segment(gw, src, q, rres, L.fptr(seg), s0, n, st),
+                        "sw_ssg2d_adjoint_segment")
         out = torch.empty((3, nz, nx), dtype=torch.float32, device=self.device)
         L.check(lib.sw_ssg2d_model_gradient(self.desc(work), q, L.fptr(self.vp_d), L.fptr(self.vs_d),
                                             L.fptr(out[0]), L.fptr(out[1]), L.fptr(out[2]), st),
                 "sw_ssg2d_model_gradient")
-        del plane
         return tuple(out[i].t() for i in range(3))
 
     def _adj_desc(self, adj, tmp, acc) -> "L.SwAdj2D":

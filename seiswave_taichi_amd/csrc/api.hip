@@ -478,6 +478,51 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
     return launch_adj2d_model_gradient(g, adj, vp, vs, gvp, gvs, grho, as_stream(stream));
 }
 
+int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_adj2d *adj,
+                             const sw_receivers *residual, float *work, int32_t it0, int32_t n, void *stream)
+{
+    int rc = check_adj2d(g, adj);
+    if (rc) return rc;
+    SW_REQUIRE(src && residual && work, "null source, residual or work buffer");
+    SW_REQUIRE(n >= 0 && it0 >= 0, "bad step range");
+    if (n == 0) return SW_OK;
+    SW_REQUIRE(src->kind == SW_SRC_NONE || src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE,
+               "the adjoint covers stress sources (explosive or moment tensor)");
+    if ((rc = check_source2d(g, src, it0)) || (rc = check_source2d(g, src, it0 + n - 1))) return rc;
+    if ((rc = check_receivers(residual, it0)) || (rc = check_receivers(residual, it0 + n - 1))) return rc;
+    SW_REQUIRE(residual->ncomp <= 16, "too many residual components");
+    for (int c = 0; c < residual->ncomp; ++c)
+        SW_REQUIRE(fields2d(g).f[residual->comp[c]] != nullptr, "component %d does not exist in 2-D",
+                   residual->comp[c]);
+    const size_t P = (size_t)g->nx * g->nz, S = 15 * P;
+    SW_REQUIRE(g->vz == g->vx + P && g->sxx == g->vx + 2 * P && g->szz == g->vx + 3 * P &&
+                   g->sxz == g->vx + 4 * P && g->mem == g->vx + 5 * P,
+               "the state must be one block of 15 planes (vx vz sxx szz sxz mem) starting at vx");
+    hipStream_t st = as_stream(stream);
+    auto state_at = [&](float *base) {
+        sw_ssg2d e = *g;
+        e.vx = base; e.vz = base + P; e.sxx = base + 2 * P; e.szz = base + 3 * P; e.sxz = base + 4 * P;
+        e.mem = base + 5 * P;
+        return e;
+    };
+    for (int t = 0; t < n; ++t) {
+        if (src->kind != SW_SRC_NONE) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it0 + t);
+        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t) * S, g->vx, S * sizeof(float), hipMemcpyDeviceToDevice, st));
+        if ((rc = launch_ssg2d_velocity(g, st))) return rc;
+        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t + 1) * S, g->vx, S * sizeof(float), hipMemcpyDeviceToDevice,
+                                  st));
+        if ((rc = launch_ssg2d_stress(g, st))) return rc;
+    }
+    for (int t = n - 1; t >= 0; --t) {
+        if (residual->nrec && (rc = launch_adj2d_inject(adj, residual, it0 + t, st))) return rc;
+        const sw_ssg2d mid = state_at(work + (2 * (size_t)t + 1) * S);
+        if ((rc = launch_adj2d_stress(&mid, adj, st))) return rc;
+        const sw_ssg2d pre = state_at(work + (2 * (size_t)t) * S);
+        if ((rc = launch_adj2d_velocity(&pre, adj, st))) return rc;
+    }
+    return launch_status("ssg2d adjoint segment");
+}
+
 static int check_band(const sw_ssg3d *g, const sw_band *b)
 {
     SW_REQUIRE(b != nullptr, "null band");

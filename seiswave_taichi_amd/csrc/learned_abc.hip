@@ -1,6 +1,6 @@
 // Learned thin absorbing boundary: a per-boundary-column MLP that corrects
-// the fields next to a thin CPML (no counterpart in the reference; the
-// north-star extension that replaces most of a wide CPML).
+// the fields next to a thin CPML (no counterpart in the reference; it
+// replaces most of a wide CPML).
 //
 // Geometry.  The "interior" is the box of cells outside every CPML slab.  A
 // band is the D = 4 outermost interior cells along one face normal; each band
@@ -20,6 +20,8 @@
 //   y = (W3^T tanh(W2^T tanh(W1^T (u / s) + b1) + b2) + b3) * out_scale * s
 // (y = 0 where s = 0).  The training entry points see samples already divided
 // by s: their forward pass is the bracket applied to x / in_scale.
+#include <algorithm>
+
 #include "sw_internal.h"
 
 namespace sw {
@@ -102,101 +104,136 @@ __global__ void k_band_gather(Fields9 F, int nx, int ny, BandGeo q, float *out)
 }
 
 // Fused inference: gather, MLP, correct the fields, keep the corrected band.
-// Blocks never straddle faces: block b serves face blk_face[b].
+//
+// A block of 256 threads works on tiles of 64 columns of one face at a time:
+// the features go to LDS, and each layer is a 64 x n_out product in which
+// thread (column c = t % 64, group g = t / 64) accumulates a quarter of the
+// outputs of column c.  The weights of the face sit in LDS and are broadcast
+// (all lanes of a wave read the same weight); activations are re-staged in
+// LDS between layers, so no thread holds more than 16 accumulators.  Blocks
+// take contiguous ranges of face-major tiles and reload weights only when
+// the face changes.
 constexpr int kCorrectThreads = 256;
+constexpr int kTile = 64;
+constexpr int kXS = kIn + 1;  // padded LDS row strides (odd: conflict-free column access)
+constexpr int kHS = kH + 1;
 
-struct FaceBlocks {
-    int blk0[7];  // first block of each face
+struct TileMap {
+    int tile0[7];  // first tile of each face (face-major order)
 };
 
-__global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, FaceBlocks fb,
+__global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, TileMap tm,
                                                                   const float *params, float *prev)
 {
     __shared__ float w[kParams];
-    int f = 0;
+    __shared__ float xs[kTile * kXS];  // features, then h2
+    __shared__ float hs[kTile * kHS];  // h1
+    __shared__ float red[4][kTile];
+    __shared__ float scale[kTile];
+    const int ntile = tm.tile0[6];
+    const int t0 = (int)(((long)blockIdx.x * ntile) / gridDim.x);
+    const int t1 = (int)(((long)(blockIdx.x + 1) * ntile) / gridDim.x);
+    const int c = threadIdx.x % kTile, grp = threadIdx.x / kTile;
+    int loaded = -1;
+    for (int tile = t0; tile < t1; ++tile) {
+        int f = 0;
 #pragma unroll
-    for (int t = 1; t < 6; ++t) f += ((int)blockIdx.x >= fb.blk0[t]);
-    const float *P = params + (long)f * kParams;
-    for (int t = threadIdx.x; t < kParams; t += blockDim.x) w[t] = P[t];
-    __syncthreads();
-    const int r = ((int)blockIdx.x - fb.blk0[f]) * blockDim.x + threadIdx.x;
-    if (r >= q.ncol[f]) return;
-    const int c = q.col0[f] + r;
-    const int na = lateral_a_extent(q, f);
-    const int a = r % na, b = r / na;
-    const float *W1 = w, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH, *B3 = W3 + kH * kOut;
-    const float *IS = B3 + kOut, *OS = IS + kIn;
-    float cur[kOut], x[kIn];
-    long pc[kD];
-#pragma unroll
-    for (int d = 0; d < kD; ++d) {
-        int i, j, k;
-        band_cell(q, f, a, b, d, i, j, k);
-        pc[d] = ((long)k * ny + j) * nx + i;
-#pragma unroll
-        for (int t = 0; t < kF; ++t) cur[d * kF + t] = F.f[t][pc[d]];
-    }
-    // features: the band after the step and its change over the step
-#pragma unroll
-    for (int t = 0; t < kOut; ++t) {
-        x[t] = cur[t];
-        x[kOut + t] = cur[t] - prev[(long)c * kOut + t];
-    }
-    // per-column scale: RMS of the channel-normalised features; the network
-    // sees unit-RMS input and its output is scaled back, so the correction is
-    // proportional to the wavefield (the wave equation is linear)
-    float ss = 0.f;
-#pragma unroll
-    for (int in = 0; in < kIn; ++in) {
-        x[in] = x[in] / IS[in];
-        ss += x[in] * x[in];
-    }
-    const float rs = sqrtf(ss / (float)kIn);
-    float y[kOut];
-    if (rs > 0.f) {
-        const float inv = 1.f / rs;
-        float h1[kH];
-#pragma unroll
-        for (int o = 0; o < kH; ++o) h1[o] = B1[o];
-        for (int in = 0; in < kIn; ++in) {
-            const float xi = x[in] * inv;
-#pragma unroll
-            for (int o = 0; o < kH; ++o) h1[o] += xi * W1[in * kH + o];
+        for (int u = 1; u < 6; ++u) f += (tile >= tm.tile0[u]);
+        if (f != loaded) {
+            __syncthreads();
+            const float *P = params + (long)f * kParams;
+            for (int u = threadIdx.x; u < kParams; u += blockDim.x) w[u] = P[u];
+            loaded = f;
         }
+        const float *W1 = w, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH;
+        const float *B3 = W3 + kH * kOut, *IS = B3 + kOut, *OS = IS + kIn;
+        const int r = (tile - tm.tile0[f]) * kTile + c;  // column within the face
+        const bool live = r < q.ncol[f];
+        const int col = q.col0[f] + r;
+        const int na = lateral_a_extent(q, f);
+        // 1. gather: thread (c, grp) takes band depth d = grp of column c
+        long pc = 0;
+        float ss = 0.f;
+        {
+            const int d = grp;
+            if (live) {
+                int i, j, k;
+                band_cell(q, f, r % na, r / na, d, i, j, k);
+                pc = ((long)k * ny + j) * nx + i;
+            }
+            __syncthreads();  // the previous tile is done with xs / hs
 #pragma unroll
-        for (int o = 0; o < kH; ++o) h1[o] = tanhf(h1[o]);
-        float h2[kH];
-#pragma unroll
-        for (int o = 0; o < kH; ++o) h2[o] = B2[o];
-        for (int in = 0; in < kH; ++in) {
-            const float hi = h1[in];
-#pragma unroll
-            for (int o = 0; o < kH; ++o) h2[o] += hi * W2[in * kH + o];
+            for (int u = 0; u < kF; ++u) {
+                const int o = d * kF + u;
+                const float cur = live ? F.f[u][pc] : 0.f;
+                const float pv = live ? prev[(long)col * kOut + o] : 0.f;
+                const float x0 = cur / IS[o];
+                const float x1 = (cur - pv) / IS[kOut + o];
+                xs[c * kXS + o] = x0;
+                xs[c * kXS + kOut + o] = x1;
+                ss += x0 * x0 + x1 * x1;
+            }
+            red[grp][c] = ss;
         }
+        __syncthreads();
+        if (grp == 0) scale[c] = sqrtf((red[0][c] + red[1][c] + red[2][c] + red[3][c]) / (float)kIn);
+        __syncthreads();
+        const float rs = scale[c];
+        const float inv = rs > 0.f ? 1.f / rs : 0.f;
+        // 2. h1 = tanh(W1^T (u / s) + b1): outputs 16 grp .. 16 grp + 15
+        {
+            float acc[16];
 #pragma unroll
-        for (int o = 0; o < kH; ++o) h2[o] = tanhf(h2[o]);
+            for (int o = 0; o < 16; ++o) acc[o] = B1[16 * grp + o];
+            for (int in = 0; in < kIn; ++in) {
+                const float xv = xs[c * kXS + in] * inv;
+                const float *wr = W1 + in * kH + 16 * grp;
 #pragma unroll
-        for (int o = 0; o < kOut; ++o) y[o] = B3[o];
-        for (int in = 0; in < kH; ++in) {
-            const float hi = h2[in];
+                for (int o = 0; o < 16; ++o) acc[o] += xv * wr[o];
+            }
 #pragma unroll
-            for (int o = 0; o < kOut; ++o) y[o] += hi * W3[in * kOut + o];
+            for (int o = 0; o < 16; ++o) hs[c * kHS + 16 * grp + o] = tanhf(acc[o]);
         }
+        __syncthreads();
+        // 3. h2 = tanh(W2^T h1 + b2), staged in xs
+        {
+            float acc[16];
 #pragma unroll
-        for (int o = 0; o < kOut; ++o) y[o] = y[o] * OS[o] * rs;
-    } else {
+            for (int o = 0; o < 16; ++o) acc[o] = B2[16 * grp + o];
+            for (int in = 0; in < kH; ++in) {
+                const float hv = hs[c * kHS + in];
+                const float *wr = W2 + in * kH + 16 * grp;
 #pragma unroll
-        for (int o = 0; o < kOut; ++o) y[o] = 0.f;
+                for (int o = 0; o < 16; ++o) acc[o] += hv * wr[o];
+            }
+#pragma unroll
+            for (int o = 0; o < 16; ++o) xs[c * kXS + 16 * grp + o] = tanhf(acc[o]);
+        }
+        __syncthreads();
+        // 4. y = (W3^T h2 + b3) * out_scale * s: outputs 9 grp .. 9 grp + 8,
+        //    i.e. the 9 fields of band depth grp; add to the band, keep it
+        {
+            float acc[kF];
+#pragma unroll
+            for (int o = 0; o < kF; ++o) acc[o] = B3[kF * grp + o];
+            for (int in = 0; in < kH; ++in) {
+                const float hv = xs[c * kXS + in];
+                const float *wr = W3 + in * kOut + kF * grp;
+#pragma unroll
+                for (int o = 0; o < kF; ++o) acc[o] += hv * wr[o];
+            }
+            if (live) {
+#pragma unroll
+                for (int u = 0; u < kF; ++u) {
+                    const int o = kF * grp + u;
+                    const float y = rs > 0.f ? acc[u] * OS[o] * rs : 0.f;
+                    const float v = F.f[u][pc] + y;
+                    F.f[u][pc] = v;
+                    prev[(long)col * kOut + o] = v;
+                }
+            }
+        }
     }
-#pragma unroll
-    for (int d = 0; d < kD; ++d)
-#pragma unroll
-        for (int t = 0; t < kF; ++t) {
-            const int o = d * kF + t;
-            const float v = cur[o] + y[o];
-            F.f[t][pc[d]] = v;
-            prev[(long)c * kOut + o] = v;
-        }
 }
 
 // ------------------------------------------------------------ training
@@ -301,12 +338,14 @@ int launch_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, hipStrea
 int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, hipStream_t st)
 {
     const BandGeo q = band_geo(*b);
-    FaceBlocks fb;
-    fb.blk0[0] = 0;
-    for (int f = 0; f < 6; ++f) fb.blk0[f + 1] = fb.blk0[f] + (int)nblk(q.ncol[f], kCorrectThreads);
-    if (fb.blk0[6] == 0) return SW_OK;
-    hipLaunchKernelGGL(k_band_correct, dim3(fb.blk0[6]), dim3(kCorrectThreads), 0, st, fields9(*g), g->nx, g->ny, q,
-                       fb, params, prev);
+    TileMap tm;
+    tm.tile0[0] = 0;
+    for (int f = 0; f < 6; ++f) tm.tile0[f + 1] = tm.tile0[f] + (q.ncol[f] + kTile - 1) / kTile;
+    if (tm.tile0[6] == 0) return SW_OK;
+    // two resident blocks per CU (80 KB of LDS each), a few tiles per block
+    const int nblock = std::min(tm.tile0[6], 512);
+    hipLaunchKernelGGL(k_band_correct, dim3(nblock), dim3(kCorrectThreads), 0, st, fields9(*g), g->nx, g->ny, q, tm,
+                       params, prev);
     return launch_status("band correction");
 }
 

@@ -5,7 +5,7 @@
 //
 // Execution: each thread owns one (i, j) column and marches along z over a
 // chunk of planes; a block takes 256 consecutive columns of a box in x-fastest
-// order.  On large grids a sweep is three launches over boxes of full-width
+// order.  On large grids a sweep is five launches over boxes of full-width
 // rows, instantiated with the CPML of only the axes whose slabs they meet (x;
 // x and y; x, y and z), so the bulk of the grid runs with fewer registers and
 // more waves in flight; on grids where the CPML shell is a large part, one
@@ -47,11 +47,10 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb)
     return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
-// A launch covers up to kMaxBox boxes of cells: columns [i0, i0+w) x
-// [j0, j0+h) over planes [k0, k1).  Each box is cut into blocks of kThreads
-// consecutive columns (x fastest, so no lane idles on a ragged x edge and a
-// wave reads at most two row segments per tap) times chunks of cz planes.
-constexpr int kMaxBox = 2;
+// A launch covers one box of cells: columns [i0, i0+w) x [j0, j0+h) over
+// planes [k0, k1).  The box is cut into blocks of kThreads consecutive
+// columns (x fastest, so no lane idles on a ragged x edge and a wave reads at
+// most two row segments per tap) times chunks of cz planes.
 constexpr int kThreads = 256;
 
 struct Box {
@@ -59,9 +58,8 @@ struct Box {
 };
 
 struct Geo {
-    int nbox, cz;
-    Box box[kMaxBox];
-    int blk0[kMaxBox + 1];  // first block of each box; blk0[nbox] = total
+    Box box;
+    int cz, nblock;
 };
 
 struct Cell {
@@ -71,17 +69,8 @@ struct Cell {
 __device__ __forceinline__ Cell box_cell(const Geo &geo)
 {
     const int b = xcd_logical_block(blockIdx.x, gridDim.x);
-    // wave-uniform selection of the box (no dynamic indexing of the argument)
-    Box B = geo.box[0];
-    int b0 = 0;
-#pragma unroll
-    for (int t = 1; t < kMaxBox; ++t)
-        if (t < geo.nbox && b >= geo.blk0[t]) {
-            B = geo.box[t];
-            b0 = geo.blk0[t];
-        }
-    const int lb = b - b0;
-    const int cb = lb % B.ncb, zb = lb / B.ncb;
+    const Box &B = geo.box;
+    const int cb = b % B.ncb, zb = b / B.ncb;
     const int t = cb * kThreads + (int)threadIdx.x;
     Cell c;
     if (t >= B.w * B.h) {
@@ -594,86 +583,90 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
 #endif
 
 // Cells a sweep updates: [N, n-N) on every axis, in one launch (PM = 7) or
-// in three launches whose boxes all keep full-width x rows (long coalesced
+// in five launches whose boxes all keep full-width x rows (long coalesced
 // segments):
 //   PM = 1: planes and rows outside the z and y slabs (x slab per lane);
-//   PM = 3: rows inside a y slab, planes outside the z slabs;
-//   PM = 7: planes inside a z slab.
+//   PM = 3: rows inside a y slab, planes outside the z slabs (2 boxes);
+//   PM = 7: planes inside a z slab (2 boxes).
 // The mask PM names the axes whose CPML the kernel instantiates; dropping z
 // and y from the bulk of the grid frees the registers of their memory
 // variables and coefficients, so more waves are in flight there.
 struct Plan {
-    Geo g[3];  // PM = 1, 3, 7
+    int n = 0;
+    int pm[5];
+    Geo g[5];
 };
-
-void add_box(Geo &geo, int i0, int i1, int j0, int j1, int k0, int k1)
-{
-    if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
-    Box &b = geo.box[geo.nbox++];
-    b.i0 = i0; b.w = i1 - i0;
-    b.j0 = j0; b.h = j1 - j0;
-    b.k0 = k0; b.k1 = k1;
-    b.ncb = (b.w * b.h + kThreads - 1) / kThreads;
-}
 
 // Chunk length: SW_CZ0 planes, doubled (up to 64) while the launch keeps at
 // least 32 blocks per CU, so that the last round of blocks is a small part of
 // a launch and the window warm-up (2N-1 planes per chunk) a small part of
 // large grids.
-void finish(Geo &geo)
+void add_box(Plan &p, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
 {
-    auto blocks = [&](int cz) {
-        long n = 0;
-        for (int t = 0; t < geo.nbox; ++t)
-            n += (long)geo.box[t].ncb * ((geo.box[t].k1 - geo.box[t].k0 + cz - 1) / cz);
-        return n;
-    };
+    if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
+    Geo &geo = p.g[p.n];
+    p.pm[p.n++] = pm;
+    Box &b = geo.box;
+    b.i0 = i0; b.w = i1 - i0;
+    b.j0 = j0; b.h = j1 - j0;
+    b.k0 = k0; b.k1 = k1;
+    b.ncb = (b.w * b.h + kThreads - 1) / kThreads;
+    auto blocks = [&](int cz) { return (long)b.ncb * ((k1 - k0 + cz - 1) / cz); };
     int cz = SW_CZ0;
     while (cz < 64 && blocks(2 * cz) >= 32L * 256) cz *= 2;
     geo.cz = cz;
-    geo.blk0[0] = 0;
-    for (int t = 0; t < geo.nbox; ++t)
-        geo.blk0[t + 1] = geo.blk0[t] + geo.box[t].ncb * ((geo.box[t].k1 - geo.box[t].k0 + cz - 1) / cz);
+    geo.nblock = (int)blocks(cz);
 }
 
 Plan make_plan(const sw_ssg3d &g, int N)
 {
-    Plan p{};
+    Plan p;
     const int x0 = N, x1 = g.nx - N, y0 = N, y1 = g.ny - N, z0 = N, z1 = g.nz - N;
     const int yl = std::max(y0, g.py.lo), yh = std::min(y1, g.py.hi);
     const int zl = std::max(z0, g.pz.lo), zh = std::min(z1, g.pz.hi);
-    // Three launches pay off only when the bulk box dominates: each extra
+    // Several launches pay off only when the bulk box dominates: each extra
     // launch adds a ramp-up and a tail (measured on MI355X: a 200^3 grid with
     // 20-cell CPML, bulk 64 % of the cells, runs faster as one launch; the
-    // 600x600x400 grid, bulk 84 %, 5 % faster as three).
-    const double bulk = (yl < yh && zl < zh)
-                            ? (double)(x1 - x0) * (yh - yl) * (zh - zl) / ((double)(x1 - x0) * (y1 - y0) * (z1 - z0))
-                            : 0.0;
+    // 600x600x400 grid, bulk 84 %, 5 % faster split).
+    const double bulk = (yl < yh && zl < zh) ? (double)(yh - yl) * (zh - zl) / ((double)(y1 - y0) * (z1 - z0)) : 0.0;
     if (bulk >= 0.75) {
-        add_box(p.g[0], x0, x1, yl, yh, zl, zh);
-        add_box(p.g[1], x0, x1, y0, yl, zl, zh);
-        add_box(p.g[1], x0, x1, yh, y1, zl, zh);
-        add_box(p.g[2], x0, x1, y0, y1, z0, zl);
-        add_box(p.g[2], x0, x1, y0, y1, zh, z1);
+        add_box(p, 1, x0, x1, yl, yh, zl, zh);
+        add_box(p, 3, x0, x1, y0, yl, zl, zh);
+        add_box(p, 3, x0, x1, yh, y1, zl, zh);
+        add_box(p, 7, x0, x1, y0, y1, z0, zl);
+        add_box(p, 7, x0, x1, y0, y1, zh, z1);
     } else {
-        add_box(p.g[2], x0, x1, y0, y1, z0, z1);
+        add_box(p, 7, x0, x1, y0, y1, z0, z1);
     }
-    for (Geo &geo : p.g) finish(geo);
     return p;
 }
 
-template <int N, int CLS, int PM>
-void launch_velocity(const sw_ssg3d &g, const Geo &geo, hipStream_t st)
+template <int N, int CLS>
+void launch_velocity(const sw_ssg3d &g, const Plan &p, hipStream_t st)
 {
-    if (geo.nbox)
-        hipLaunchKernelGGL((k_velocity<N, CLS, PM>), dim3(geo.blk0[geo.nbox]), dim3(kThreads), 0, st, g, geo);
+    for (int t = 0; t < p.n; ++t) {
+        const dim3 grd(p.g[t].nblock), blk(kThreads);
+        if (p.pm[t] == 1)
+            hipLaunchKernelGGL((k_velocity<N, CLS, 1>), grd, blk, 0, st, g, p.g[t]);
+        else if (p.pm[t] == 3)
+            hipLaunchKernelGGL((k_velocity<N, CLS, 3>), grd, blk, 0, st, g, p.g[t]);
+        else
+            hipLaunchKernelGGL((k_velocity<N, CLS, 7>), grd, blk, 0, st, g, p.g[t]);
+    }
 }
 
-template <int N, int PM>
-void launch_stress(const sw_ssg3d &g, const Geo &geo, hipStream_t st)
+template <int N>
+void launch_stress(const sw_ssg3d &g, const Plan &p, hipStream_t st)
 {
-    if (geo.nbox)
-        hipLaunchKernelGGL((k_stress<N, PM>), dim3(geo.blk0[geo.nbox]), dim3(kThreads), 0, st, g, geo);
+    for (int t = 0; t < p.n; ++t) {
+        const dim3 grd(p.g[t].nblock), blk(kThreads);
+        if (p.pm[t] == 1)
+            hipLaunchKernelGGL((k_stress<N, 1>), grd, blk, 0, st, g, p.g[t]);
+        else if (p.pm[t] == 3)
+            hipLaunchKernelGGL((k_stress<N, 3>), grd, blk, 0, st, g, p.g[t]);
+        else
+            hipLaunchKernelGGL((k_stress<N, 7>), grd, blk, 0, st, g, p.g[t]);
+    }
 }
 
 }  // namespace
@@ -682,27 +675,17 @@ int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st)
 {
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
-        if (g->cls == SW_CLASS_TENSOR) {
-            launch_velocity<N, SW_CLASS_TENSOR, 1>(*g, p.g[0], st);
-            launch_velocity<N, SW_CLASS_TENSOR, 3>(*g, p.g[1], st);
-            launch_velocity<N, SW_CLASS_TENSOR, 7>(*g, p.g[2], st);
-        } else {
-            launch_velocity<N, SW_CLASS_CPML, 1>(*g, p.g[0], st);
-            launch_velocity<N, SW_CLASS_CPML, 3>(*g, p.g[1], st);
-            launch_velocity<N, SW_CLASS_CPML, 7>(*g, p.g[2], st);
-        }
+        if (g->cls == SW_CLASS_TENSOR)
+            launch_velocity<N, SW_CLASS_TENSOR>(*g, p, st);
+        else
+            launch_velocity<N, SW_CLASS_CPML>(*g, p, st);
     });
     return launch_status("ssg3d velocity");
 }
 
 int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st)
 {
-    SW_DISPATCH_ORDER(g->order, {
-        const Plan p = make_plan(*g, N);
-        launch_stress<N, 1>(*g, p.g[0], st);
-        launch_stress<N, 3>(*g, p.g[1], st);
-        launch_stress<N, 7>(*g, p.g[2], st);
-    });
+    SW_DISPATCH_ORDER(g->order, { launch_stress<N>(*g, make_plan(*g, N), st); });
     return launch_status("ssg3d stress");
 }
 

@@ -143,6 +143,11 @@ def courant(vp_max, d, dt, order=2):
     return float(vp_max) * dt / d * np.sqrt(3.0) * float(np.sum(np.abs(diff_coff(order))))
 
 
+# CPML profile of every learned-boundary run, independent of the source: the
+# networks learn the error of one particular thin layer.
+PML_FREQ = 15.0
+
+
 def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
     from .Elastic3D_SSG_Tensor import ElasticWAVE
     cr = courant(vp.max(), d, dt)
@@ -154,7 +159,7 @@ def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
                     np.zeros((1, 1), np.int64) + cell[1], np.zeros((1, 1), np.int64) + cell[2], mt3, 1.0, nt, 2, f,
                     source={"kind": "moment", "cell": cell, "mt": mt, "wavelet": amps})
     p = {"vp_max": float(vp.max()), "pml_x_thick": npml * d, "pml_y_thick": npml * d, "pml_z_thick": npml * d,
-         "Rcoef": 1e-6, "theta": 1, "alpha_max_pml": np.pi * f, "kmax_pml": 1.0}
+         "Rcoef": 1e-6, "theta": 1, "alpha_max_pml": np.pi * PML_FREQ, "kmax_pml": 1.0}
     w.SetADEPML3D([True] * 6, p)
     return w
 
@@ -164,7 +169,7 @@ def grid_size(n_int, npml, s=2):
     return n_int + 2 * s + 2 * npml + 1
 
 
-def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20, keep=0.2):
+def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=None, thin=4, wide=20, keep=0.2):
     """One teacher-forced run.
 
     A wide-CPML run of a random model is the reference; a thin-CPML run of the
@@ -172,8 +177,11 @@ def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=15.0, thin=4, wide=
     every band column: features [cur, cur - prev] (cur = thin band after the
     step, prev = the band before it) and target = reference band - cur.
     Returns per-face (X [m, 72], T [m, 36]) on the device, a random ``keep``
-    fraction of the columns of every step after the first 10."""
+    fraction of the columns of every step after the first 10.  The source
+    peak frequency is drawn from 10-20 Hz unless ``f`` is given."""
     lib = _sig()
+    if f is None:
+        f = float(rng.uniform(10.0, 20.0))
     nw, nt_ = grid_size(n_int, wide), grid_size(n_int, thin)
     off = wide - thin
     vs, vp, rho = random_layered_model((nw, nw, nw), rng)
@@ -181,7 +189,7 @@ def collect_run(rng, n_int=40, steps=160, d=10.0, dt=1e-3, f=15.0, thin=4, wide=
     src = tuple(int(v) for v in rng.integers(lo_w + 6, lo_w + n_int - 6, 3))
     m = rng.standard_normal(6)
     m /= np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2 + 2 * (m[3] ** 2 + m[4] ** 2 + m[5] ** 2))
-    amps = (1e6 * ricker2(np.arange(steps), dt, 1.2 / f, f, 1.0)).astype(np.float32)
+    amps = (1e6 * ricker2(np.arange(steps), dt, 1.0 / f, f, 1.0)).astype(np.float32)
     W = _sim(vs, vp, rho, d, dt, src, m, amps, wide, f)
     sl = slice(off, off + nt_)
     T_ = _sim(vs[sl, sl, sl].copy(), vp[sl, sl, sl].copy(), rho[sl, sl, sl].copy(), d, dt,

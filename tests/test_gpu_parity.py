@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the drop-in classes and the C ABI) against
 the f32 CPU oracle on the same seeded inputs.
 
-Tolerance: the north-star bar is a per-trace relative L2 <= 1e-5.  Both sides
+Tolerance: the parity bar is a per-trace relative L2 <= 1e-5.  Both sides
 use the reference's term order with no fused multiply-add, so the traces are
 expected to agree far below that; the strict tests also assert it.
 """
@@ -127,7 +127,7 @@ def test_spml_class_matches_oracle(order):
 
 
 def test_oracle_own_wavelet_within_tolerance():
-    """The oracle with its own (glibc expf) wavelet: within the north-star tolerance."""
+    """The oracle with its own (glibc expf) wavelet: within the 1e-5 per-trace tolerance."""
     from seiswave_taichi_amd.Elastic3D_SSG_Tensor import ElasticWAVE
     shape, order, steps, f, dt = (26, 26, 26), 2, 80, 15.0, 5e-4
     d = (10.0,) * 3

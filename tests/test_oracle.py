@@ -13,8 +13,8 @@ from oracle import oracle as O
 from oracle.numpy_ref import NumpyRef2D, NumpyRef3D
 from tests.cases import homogeneous, layered_model, pml_dict, rel_l2
 
-# Taylor staggered coefficients as exact rationals (SURVEY table a1; standard
-# result, e.g. Levander 1988 for N=2).
+# Taylor staggered coefficients as exact rationals (src/Elastic3D_SSG_CPML.py:311-322
+# solves for them; standard result, e.g. Levander 1988 for N=2).
 RATIONALS = {
     1: [1.0],
     2: [9 / 8, -1 / 24],
