@@ -192,7 +192,8 @@ typedef struct sw_adj2d {
 
 int sw_ssg2d_adjoint_stress(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream);
 int sw_ssg2d_adjoint_velocity(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream);
-/* adds residual[it] (an sw_receivers whose data holds residuals) at the receivers */
+/* adds residual[it] (an sw_receivers whose data holds residuals) at the receivers;
+ * receivers sharing a cell add up in no fixed order (f32 atomics) */
 int sw_ssg2d_adjoint_inject(const sw_ssg2d *g, const sw_adj2d *adj, const sw_receivers *residual,
                             int32_t it, void *stream);
 /* gvp, gvs, grho [nz][nx] from the accumulated coefficient gradients */

@@ -228,14 +228,17 @@ __global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q)
 }
 
 // Residual injection: adjoint of the receiver gather data[it] = field[idx].
+// One thread per (component, receiver).  Receivers at the same cell add up;
+// with such duplicates their sum order is not fixed (hardware f32 atomics).
 __global__ void k_adj_inject(sw_adj2d q, sw_receivers r, int it)
 {
-    const int comp = threadIdx.x;
-    if (comp >= r.ncomp) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= r.ncomp * r.nrec) return;
+    const int comp = t / r.nrec, k = t % r.nrec;
     float *fld = r.comp[comp] == 0 ? q.vx : (r.comp[comp] == 2 ? q.vz : (r.comp[comp] == 3 ? q.sxx :
                  (r.comp[comp] == 5 ? q.szz : q.sxz)));
-    const float *res = r.data + ((long)it * r.ncomp + comp) * r.nrec;
-    for (int k = 0; k < r.nrec; ++k) fld[r.idx[k]] += res[k];  // serial: duplicate receivers add up
+    const float res = r.data[((long)it * r.ncomp + comp) * r.nrec + k];
+    atomicAdd(fld + r.idx[k], res);
 }
 
 // Model gradient from the accumulated coefficient gradients:
@@ -307,7 +310,8 @@ int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
 
 int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(16), 0, st, *q, *r, it);
+    const int n = r->ncomp * r->nrec;
+    hipLaunchKernelGGL(k_adj_inject, dim3((n + 255) / 256), dim3(256), 0, st, *q, *r, it);
     return launch_status("2-D adjoint residual injection");
 }
 
