@@ -53,9 +53,11 @@ extern "C" {
 /* One axis of PML coefficients as SetADEPML3D leaves them
  * (src/Elastic3D_SSG_CPML.py:364-522), all device arrays of length n.
  * a/b/k drive the CPML memory variables; d drives the split-field PML.
- * Memory variables of this axis exist only for indices [0,lo) and [hi,n):
- * outside that range the coefficients are the identity (a = 0, k = 1) and the
- * reference's memory variable stays exactly 0. */
+ * CPML: memory variables of this axis exist only for indices [0,lo) and
+ * [hi,n): outside that range the coefficients are the identity (a = 0, k = 1)
+ * and the reference's memory variable stays exactly 0.
+ * Split-field PML: in[i] and in_h[i] are both 0 for every i in [lo,hi) (the
+ * split-free interior of this axis; lo = hi = 0 if there is none). */
 typedef struct sw_axis {
     const float *a, *b, *k;       /* integer points */
     const float *a_h, *b_h, *k_h; /* half points    */

@@ -22,6 +22,19 @@ from .pml import axis_pml, check_parameter
 WAVEFIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
 
 
+def split_free_range(ax):
+    """[lo, hi): the indices of an axis where neither the integer nor the half
+    point is in the split-field PML (both flags 0); (0, 0) if there are none."""
+    free = ~(np.asarray(ax.inside, bool) | np.asarray(ax.inside_h, bool))
+    idx = np.nonzero(free)[0]
+    if idx.size == 0:
+        return 0, 0
+    lo, hi = int(idx[0]), int(idx[-1]) + 1
+    if not free[lo:hi].all():
+        raise ValueError("split-field PML flags are not a pair of slabs")
+    return lo, hi
+
+
 def _check_fieldtype(fieldtype):
     name = getattr(fieldtype, "__name__", None) or str(fieldtype)
     if fieldtype in (None, np.float32, torch.float32) or "f32" in name or "float32" in name:
@@ -149,7 +162,7 @@ class SSG3DEngine:
             a.in_ = C.cast(C.c_void_p(flags.data_ptr()), L._u8p)
             a.in_h = C.cast(C.c_void_p(flags.data_ptr() + n), L._u8p)
             if self.cls == L.SW_CLASS_SPML:
-                a.lo, a.hi = 0, n
+                a.lo, a.hi = split_free_range(ax)
             else:
                 a.lo, a.hi = ax.lo, ax.hi
             setattr(g, name, a)

@@ -5,11 +5,18 @@
 // split by derivative direction and advanced with a Crank-Nicolson step on the
 // damping d only; the component is the sum of its splits.  Elsewhere the plain
 // update runs.  Split arrays are full-grid; only PML-region cells touch them.
+#include <algorithm>
+
 #include "sw_internal.h"
 
 namespace sw {
 
 namespace {
+
+// A box of cells [i0,i1) x [j0,j1) x [k0,k1).
+struct SBox {
+    int i0, i1, j0, j1, k0, k1;
+};
 
 __device__ __forceinline__ float cn_split(float *q, float n, float d, float incr)
 {
@@ -19,13 +26,13 @@ __device__ __forceinline__ float cn_split(float *q, float n, float d, float incr
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_velocity_spml(sw_ssg3d g)
+__global__ void __launch_bounds__(256) k_velocity_spml(sw_ssg3d g, SBox bx)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
-    const int k = blockIdx.z + N;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + bx.i0;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + bx.j0;
+    const int k = blockIdx.z + bx.k0;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i >= nx - N || j >= ny - N || k >= nz - N) return;
+    if (i >= bx.i1 || j >= bx.j1) return;
     const long SY = nx, SZ = (long)nx * ny, NC = SZ * nz;
     const long p = k * SZ + j * SY + i;
     const float dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt, hdt = 0.5f * g.dt;
@@ -85,13 +92,13 @@ __global__ void __launch_bounds__(256) k_velocity_spml(sw_ssg3d g)
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_stress_spml(sw_ssg3d g)
+__global__ void __launch_bounds__(256) k_stress_spml(sw_ssg3d g, SBox bx)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
-    const int k = blockIdx.z + N;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + bx.i0;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + bx.j0;
+    const int k = blockIdx.z + bx.k0;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i >= nx - N || j >= ny - N || k >= nz - N) return;
+    if (i >= bx.i1 || j >= bx.j1) return;
     const long SY = nx, SZ = (long)nx * ny, NC = SZ * nz;
     const long p = k * SZ + j * SY + i;
     const float dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt, hdt = 0.5f * g.dt;
@@ -171,28 +178,76 @@ __global__ void __launch_bounds__(256) k_stress_spml(sw_ssg3d g)
     }
 }
 
-inline dim3 box_grid(const sw_ssg3d &g, int N, dim3 blk)
+constexpr int kSX = 64, kSY = 4;
+
+// Split-free interior box (plain updates, z-marched in ssg3d.hip) and the
+// shell of up to six boxes around it (thread-per-cell kernels above).
+struct SPlan {
+    int n = 0;
+    SBox shell[6];
+    SBox inner{0, 0, 0, 0, 0, 0};
+};
+
+void add(SPlan &p, int i0, int i1, int j0, int j1, int k0, int k1)
 {
-    const int ex = g.nx - 2 * N, ey = g.ny - 2 * N, ez = g.nz - 2 * N;
-    return dim3((ex + blk.x - 1) / blk.x, (ey + blk.y - 1) / blk.y, ez);
+    if (i1 > i0 && j1 > j0 && k1 > k0) p.shell[p.n++] = SBox{i0, i1, j0, j1, k0, k1};
+}
+
+SPlan make_splan(const sw_ssg3d &g, int N)
+{
+    SPlan p;
+    const int x0 = N, x1 = g.nx - N, y0 = N, y1 = g.ny - N, z0 = N, z1 = g.nz - N;
+    const int xl = std::max(x0, g.px.lo), xh = std::min(x1, g.px.hi);
+    const int yl = std::max(y0, g.py.lo), yh = std::min(y1, g.py.hi);
+    const int zl = std::max(z0, g.pz.lo), zh = std::min(z1, g.pz.hi);
+    if (xl < xh && yl < yh && zl < zh) {
+        p.inner = SBox{xl, xh, yl, yh, zl, zh};
+        add(p, x0, x1, y0, y1, z0, zl);
+        add(p, x0, x1, y0, y1, zh, z1);
+        add(p, x0, x1, y0, yl, zl, zh);
+        add(p, x0, x1, yh, y1, zl, zh);
+        add(p, x0, xl, yl, yh, zl, zh);
+        add(p, xh, x1, yl, yh, zl, zh);
+    } else {
+        add(p, x0, x1, y0, y1, z0, z1);
+    }
+    return p;
+}
+
+inline dim3 sgrid(const SBox &b)
+{
+    return dim3((b.i1 - b.i0 + kSX - 1) / kSX, (b.j1 - b.j0 + kSY - 1) / kSY, b.k1 - b.k0);
 }
 
 }  // namespace
 
+int launch_ssg3d_spml_interior(const sw_ssg3d *g, int i0, int i1, int j0, int j1, int k0, int k1, bool stress,
+                               hipStream_t st);
+
 int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kSX, kSY, 1);
     SW_DISPATCH_ORDER(g->order, {
-        hipLaunchKernelGGL((k_velocity_spml<N>), box_grid(*g, N, blk), blk, 0, st, *g);
+        const SPlan p = make_splan(*g, N);
+        for (int t = 0; t < p.n; ++t)
+            hipLaunchKernelGGL((k_velocity_spml<N>), sgrid(p.shell[t]), blk, 0, st, *g, p.shell[t]);
+        const SBox &b = p.inner;
+        int rc = launch_ssg3d_spml_interior(g, b.i0, b.i1, b.j0, b.j1, b.k0, b.k1, false, st);
+        if (rc) return rc;
     });
     return launch_status("ssg3d velocity (split-field PML)");
 }
 
 int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kSX, kSY, 1);
     SW_DISPATCH_ORDER(g->order, {
-        hipLaunchKernelGGL((k_stress_spml<N>), box_grid(*g, N, blk), blk, 0, st, *g);
+        const SPlan p = make_splan(*g, N);
+        for (int t = 0; t < p.n; ++t)
+            hipLaunchKernelGGL((k_stress_spml<N>), sgrid(p.shell[t]), blk, 0, st, *g, p.shell[t]);
+        const SBox &b = p.inner;
+        int rc = launch_ssg3d_spml_interior(g, b.i0, b.i1, b.j0, b.j1, b.k0, b.k1, true, st);
+        if (rc) return rc;
     });
     return launch_status("ssg3d stress (split-field PML)");
 }

@@ -340,7 +340,10 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
                 dsxz = cpml_step(dsxz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 0 * sl.nzs + oz, P.mz[0]);
             }
-            st(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
+            if (CLS == SW_CLASS_SPML)  // src/Elastic3D_SSG_SPML.py:199-204: x+y+z, times 1/rho
+                st(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * (1.f / P.rho[0]));
+            else
+                st(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
         }
         // vy at (i, j+1/2, k): :180-195
         if (vy_col && k >= N + 1) {
@@ -360,7 +363,10 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
                 dsyz = cpml_step(dsyz, P.mz[1], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 1 * sl.nzs + oz, P.mz[1]);
             }
-            st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
+            if (CLS == SW_CLASS_SPML)
+                st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * (1.f / rho_half_y));
+            else
+                st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
         }
         // vz at (i, j, k+1/2): :197-212 (Tensor :228-243)
         if (vz_col && k < nz - N - 1) {
@@ -380,7 +386,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
                 dszz = cpml_step(dszz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 2 * sl.nzs + oz, P.mz[2]);
             }
-            if (CLS == SW_CLASS_TENSOR)
+            if (CLS == SW_CLASS_SPML)
+                st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * (1.f / rho_avg));
+            else if (CLS == SW_CLASS_TENSOR)
                 st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt / rho_avg);
             else
                 st(R.vz, p, P.v[2] + (dszz + dsxz + dsyz) * dt / rho_avg);
@@ -452,7 +460,7 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
     }
 }
 
-template <int N, int PM>
+template <int N, int CLS, int PM>
 __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
 {
     const Cell cl = box_cell(geo);
@@ -502,8 +510,10 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
         wvy[2 * N - 1] = P.wvy;
         // normal stresses at (i, j, k): src/Elastic3D_SSG_CPML.py:247-266
         if (nrm_col && k >= N + 1) {
-            const float lam_h = 0.5f * (P.lam[0] + P.lam[1]);
-            const float mu_h = 0.5f * (P.mu[0] + P.mu[1]);
+            // CPML/Tensor average lambda, mu over (i, i+1); SPML takes the
+            // point values (src/Elastic3D_SSG_SPML.py:300-302)
+            const float lam_h = CLS == SW_CLASS_SPML ? P.lam[0] : 0.5f * (P.lam[0] + P.lam[1]);
+            const float mu_h = CLS == SW_CLASS_SPML ? P.mu[0] : 0.5f * (P.mu[0] + P.mu[1]);
             const float lpm = lam_h + 2.f * mu_h;
             float dvxx = diff_bm<N>(P.vx_x, c) / dx;
             float dvyy = diff_bm<N>(P.vy_y, c) / dy;
@@ -661,15 +671,33 @@ void launch_stress(const sw_ssg3d &g, const Plan &p, hipStream_t st)
     for (int t = 0; t < p.n; ++t) {
         const dim3 grd(p.g[t].nblock), blk(kThreads);
         if (p.pm[t] == 1)
-            hipLaunchKernelGGL((k_stress<N, 1>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, SW_CLASS_CPML, 1>), grd, blk, 0, st, g, p.g[t]);
         else if (p.pm[t] == 3)
-            hipLaunchKernelGGL((k_stress<N, 3>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, SW_CLASS_CPML, 3>), grd, blk, 0, st, g, p.g[t]);
         else
-            hipLaunchKernelGGL((k_stress<N, 7>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, SW_CLASS_CPML, 7>), grd, blk, 0, st, g, p.g[t]);
     }
 }
 
 }  // namespace
+
+// Split-field PML class, split-free interior box only: the plain updates of
+// src/Elastic3D_SSG_SPML.py (no CPML code instantiated, PM = 0).
+int launch_ssg3d_spml_interior(const sw_ssg3d *g, int i0, int i1, int j0, int j1, int k0, int k1, bool stress,
+                               hipStream_t st)
+{
+    Plan p;
+    add_box(p, 0, i0, i1, j0, j1, k0, k1);
+    if (p.n == 0) return SW_OK;
+    const dim3 grd(p.g[0].nblock), blk(kThreads);
+    SW_DISPATCH_ORDER(g->order, {
+        if (stress)
+            hipLaunchKernelGGL((k_stress<N, SW_CLASS_SPML, 0>), grd, blk, 0, st, *g, p.g[0]);
+        else
+            hipLaunchKernelGGL((k_velocity<N, SW_CLASS_SPML, 0>), grd, blk, 0, st, *g, p.g[0]);
+    });
+    return SW_OK;
+}
 
 int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st)
 {

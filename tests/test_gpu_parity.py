@@ -126,6 +126,29 @@ def test_spml_class_matches_oracle(order):
         np.testing.assert_array_equal(w.split_field(n), o.split[i], err_msg=n)
 
 
+def test_spml_partial_pml_matches_oracle():
+    """Split-field PML on four faces only (no PML at x-low and z-top): the
+    split-free interior box then reaches the grid edge on those sides."""
+    from seiswave_taichi_amd.Elastic3D_SSG_SPML import ElasticWAVE
+    shape, steps, f, dt, order = (34, 24, 28), 70, 20.0, 5e-4, 2
+    d = (10.0, 12.0, 8.0)
+    vs, vp, rho = layered_model(shape, seed=12)
+    src, rec = (12, 11, 9), (20, 7, 5)
+    w = ElasticWAVE(vs, vp, rho, *d, dt, *src, *rec, steps, order, f, 1.0)
+    p = _pml(vp, vs, 5, d, f)
+    surf = [False, True, True, True, False, True]
+    w.SetADEPML3D(surf, p)
+    for it in range(steps):
+        w.update_SSG_SPML(it)
+    o = O.Oracle3D("spml", vs, vp, rho, *d, dt, src, [rec], steps, order, freq=f, src_scale=1.0)
+    o.set_pml(surf, p)
+    o.run(steps, amps=ricker(np.arange(steps), dt, f, 1.0))
+    _compare(w, o)
+    from seiswave_taichi_amd._classes3d import SPLIT_NAMES
+    for i, n in enumerate(SPLIT_NAMES):
+        np.testing.assert_array_equal(w.split_field(n), o.split[i], err_msg=n)
+
+
 def test_oracle_own_wavelet_within_tolerance():
     """The oracle with its own (glibc expf) wavelet: within the 1e-5 per-trace tolerance."""
     from seiswave_taichi_amd.Elastic3D_SSG_Tensor import ElasticWAVE
