@@ -139,6 +139,45 @@ __device__ __forceinline__ void taps_fm(float *w, const float *__restrict__ f, u
     for (int t = 0; t < 2 * N; ++t) w[t] = ldu(f, vb, c0 + (t - N + 1) * st);
 }
 
+// x taps are 2N consecutive floats: one 16-byte (or 8-byte) load per 4 (or 2)
+// taps instead of one load each.  The addresses are only 4-byte aligned,
+// which global vector loads accept.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+struct __attribute__((packed, aligned(4))) F4U { f4v v; };
+struct __attribute__((packed, aligned(4))) F2U { f2v v; };
+
+template <int M>
+__device__ __forceinline__ void ld_run(float *w, const float *__restrict__ f, uint32_t v, int d)
+{
+    const char *b = reinterpret_cast<const char *>(f) + v + (uint32_t)d;
+    int t = 0;
+#pragma unroll
+    for (; t + 4 <= M; t += 4) {
+        const f4v x = reinterpret_cast<const F4U *>(b + 4 * t)->v;
+        w[t] = x.x; w[t + 1] = x.y; w[t + 2] = x.z; w[t + 3] = x.w;
+    }
+#pragma unroll
+    for (; t + 2 <= M; t += 2) {
+        const f2v x = reinterpret_cast<const F2U *>(b + 4 * t)->v;
+        w[t] = x.x; w[t + 1] = x.y;
+    }
+#pragma unroll
+    for (; t < M; ++t) w[t] = *reinterpret_cast<const float *>(b + 4 * t);
+}
+
+template <int N>
+__device__ __forceinline__ void xtaps_bm(float *w, const float *__restrict__ f, uint32_t vb, int c0)
+{
+    ld_run<2 * N>(w, f, vb, c0 - N * 4);
+}
+
+template <int N>
+__device__ __forceinline__ void xtaps_fm(float *w, const float *__restrict__ f, uint32_t vb, int c0)
+{
+    ld_run<2 * N>(w, f, vb, c0 + (1 - N) * 4);
+}
+
 // Staggered differences from tap arrays, accumulated exactly as
 // src/Differential3D.py:7-44 does (0, then c[id]*(difference) in id order).
 template <int N>
@@ -248,11 +287,11 @@ __device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, con
     // vb: byte offset of (i-N, j-N, k); every load below is vb + a uniform delta
     const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
     const int c0 = N * 4 + N * SY;
-    taps_fm<N>(P.sxx, R.sxx, vb, c0, 4);
-    taps_bm<N>(P.sxy_x, R.sxy, vb, c0, 4);
+    xtaps_fm<N>(P.sxx, R.sxx, vb, c0);
+    xtaps_bm<N>(P.sxy_x, R.sxy, vb, c0);
     taps_bm<N>(P.sxy_y, R.sxy, vb, c0, SY);
     taps_fm<N>(P.syy, R.syy, vb, c0, SY);
-    taps_bm<N>(P.sxz_x, R.sxz, vb, c0, 4);
+    xtaps_bm<N>(P.sxz_x, R.sxz, vb, c0);
     taps_bm<N>(P.syz_y, R.syz, vb, c0, SY);
     P.rho[0] = ldu(R.rho, vb, c0);
     P.rho[1] = ldu(R.rho, vb, c0 + SY);
@@ -426,11 +465,11 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
 {
     const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
     const int c0 = N * 4 + N * SY;
-    taps_bm<N>(P.vx_x, R.vx, vb, c0, 4);
+    xtaps_bm<N>(P.vx_x, R.vx, vb, c0);
     taps_fm<N>(P.vx_y, R.vx, vb, c0, SY);
-    taps_fm<N>(P.vy_x, R.vy, vb, c0, 4);
+    xtaps_fm<N>(P.vy_x, R.vy, vb, c0);
     taps_bm<N>(P.vy_y, R.vy, vb, c0, SY);
-    taps_fm<N>(P.vz_x, R.vz, vb, c0, 4);
+    xtaps_fm<N>(P.vz_x, R.vz, vb, c0);
     taps_fm<N>(P.vz_y, R.vz, vb, c0, SY);
     P.lam[0] = ldu(R.lam, vb, c0);
     P.lam[1] = ldu(R.lam, vb, c0 + 4);
