@@ -106,15 +106,25 @@ __global__ void k_band_gather(Fields9 F, int nx, int ny, BandGeo q, float *out)
 // Fused inference: gather, MLP, correct the fields, keep the corrected band.
 //
 // A block of 256 threads works on tiles of 64 columns of one face at a time:
-// the features go to LDS, and each layer is a 64 x n_out product in which
-// thread (column c = t % 64, group g = t / 64) accumulates a quarter of the
-// outputs of column c.  The weights of the face sit in LDS and are broadcast
-// (all lanes of a wave read the same weight); activations are re-staged in
-// LDS between layers, so no thread holds more than 16 accumulators.  Blocks
-// take contiguous ranges of face-major tiles and reload weights only when
-// the face changes.
+// the features go to LDS, and each layer is a 64 x n_out product on the
+// matrix cores (f32 16x16x4 MFMA, full f32 arithmetic), each wave owning 16
+// columns; the weights of the face sit in LDS and activations are re-staged
+// in LDS between layers.  Blocks take contiguous ranges of face-major tiles
+// and reload weights only when the face changes.
 constexpr int kCorrectThreads = 256;
 constexpr int kTile = 64;
+typedef float mf4 __attribute__((ext_vector_type(4)));
+constexpr int kW1End = kIn * kH;                  // end of W1 in a face's parameters
+constexpr int kW2Begin = kIn * kH + kH;           // W2 follows b1
+constexpr int kW2End = kW2Begin + kH * kH;
+static_assert(kParams % 4 == 0 && kW1End % 4 == 0 && kW2Begin % 4 == 0, "16-byte weight loads");
+
+// tanh from the hardware exponential: 1 - 2 / (e^{2x} + 1); absolute error
+// ~1e-7 (relative error grows only where tanh itself is tiny).
+__device__ __forceinline__ float fast_tanh(float x)
+{
+    return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f);
+}
 constexpr int kXS = kIn + 1;  // padded LDS row strides (odd: conflict-free column access)
 constexpr int kHS = kH + 1;
 
@@ -122,12 +132,51 @@ struct TileMap {
     int tile0[7];  // first tile of each face (face-major order)
 };
 
+// One thread's share of a tile's band: depth gd of column gc (see below).
+struct BandIO {
+    int f, gc, gd, col;
+    bool live;
+    long pc;
+    float cur[kF], pv[kF];
+};
+
+// Global accesses (gather and scatter) use thread (gc, gd): on x faces the 4
+// band cells of a column are consecutive in x, so 4 neighbouring lanes take
+// the 4 depths of one column (16-byte runs); on y and z faces consecutive
+// lanes take consecutive columns, which are consecutive in x.
+__device__ __forceinline__ void fetch_band(BandIO &b, const Fields9 &F, int nx, int ny, const BandGeo &q,
+                                           const TileMap &tm, const float *prev, int tile)
+{
+    int f = 0;
+#pragma unroll
+    for (int u = 1; u < 6; ++u) f += (tile >= tm.tile0[u]);
+    const bool xface = f < 2;
+    b.f = f;
+    b.gc = xface ? (int)threadIdx.x >> 2 : (int)threadIdx.x % kTile;
+    b.gd = xface ? (int)threadIdx.x & 3 : (int)threadIdx.x / kTile;
+    const int r = (tile - tm.tile0[f]) * kTile + b.gc;  // column within the face
+    b.live = r < q.ncol[f];
+    b.col = q.col0[f] + r;
+    b.pc = 0;
+    if (b.live) {
+        const int na = lateral_a_extent(q, f);
+        int i, j, k;
+        band_cell(q, f, r % na, r / na, b.gd, i, j, k);
+        b.pc = ((long)k * ny + j) * nx + i;
+    }
+#pragma unroll
+    for (int u = 0; u < kF; ++u) {
+        b.cur[u] = b.live ? F.f[u][b.pc] : 0.f;
+        b.pv[u] = b.live ? prev[(long)b.col * kOut + b.gd * kF + u] : 0.f;
+    }
+}
+
 __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, TileMap tm,
                                                                   const float *params, float *prev)
 {
-    __shared__ float w[kParams];
+    __shared__ __attribute__((aligned(16))) float w[kParams];
     __shared__ float xs[kTile * kXS];  // features, then h2
-    __shared__ float hs[kTile * kHS];  // h1
+    __shared__ float hs[kTile * kHS];  // h1, then the correction
     __shared__ float red[4][kTile];
     __shared__ float scale[kTile];
     const int ntile = tm.tile0[6];
@@ -135,104 +184,142 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
     const int t1 = (int)(((long)(blockIdx.x + 1) * ntile) / gridDim.x);
     const int c = threadIdx.x % kTile, grp = threadIdx.x / kTile;
     int loaded = -1;
+    BandIO io, nx_io;
+    if (t0 < t1) fetch_band(io, F, nx, ny, q, tm, prev, t0);
     for (int tile = t0; tile < t1; ++tile) {
-        int f = 0;
-#pragma unroll
-        for (int u = 1; u < 6; ++u) f += (tile >= tm.tile0[u]);
+        const int f = io.f;
+        __syncthreads();  // the previous tile is done with w, xs and hs
         if (f != loaded) {
-            __syncthreads();
             const float *P = params + (long)f * kParams;
-            for (int u = threadIdx.x; u < kParams; u += blockDim.x) w[u] = P[u];
+            // W1 and W2 (64 outputs each) are stored permuted, [row][li][nb] =
+            // W[row][16 nb + li], so one 16-byte LDS read gives a lane its
+            // B operands of all 4 output blocks; the rest as in params.
+            // 16-byte global loads, 4 in flight per thread.
+            const mf4 *P4 = reinterpret_cast<const mf4 *>(P);
+#pragma unroll 4
+            for (int v = threadIdx.x; v < kParams / 4; v += blockDim.x) {
+                const mf4 x = P4[v];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int u = 4 * v + e;
+                    int dst = u;
+                    if (u < kW1End || (u >= kW2Begin && u < kW2End)) {
+                        const int base = u < kW1End ? 0 : kW2Begin;
+                        const int row = (u - base) / kH, col = (u - base) % kH;
+                        dst = base + row * kH + 4 * (col % 16) + col / 16;
+                    }
+                    w[dst] = x[e];
+                }
+            }
             loaded = f;
+            __syncthreads();
         }
         const float *W1 = w, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH;
         const float *B3 = W3 + kH * kOut, *IS = B3 + kOut, *OS = IS + kIn;
-        const int r = (tile - tm.tile0[f]) * kTile + c;  // column within the face
-        const bool live = r < q.ncol[f];
-        const int col = q.col0[f] + r;
-        const int na = lateral_a_extent(q, f);
-        // 1. gather: thread (c, grp) takes band depth d = grp of column c
-        long pc = 0;
-        float ss = 0.f;
+        // 1. features of band depth gd of column gc (fetched ahead)
         {
-            const int d = grp;
-            if (live) {
-                int i, j, k;
-                band_cell(q, f, r % na, r / na, d, i, j, k);
-                pc = ((long)k * ny + j) * nx + i;
-            }
-            __syncthreads();  // the previous tile is done with xs / hs
+            float ss = 0.f;
 #pragma unroll
             for (int u = 0; u < kF; ++u) {
-                const int o = d * kF + u;
-                const float cur = live ? F.f[u][pc] : 0.f;
-                const float pv = live ? prev[(long)col * kOut + o] : 0.f;
-                const float x0 = cur / IS[o];
-                const float x1 = (cur - pv) / IS[kOut + o];
-                xs[c * kXS + o] = x0;
-                xs[c * kXS + kOut + o] = x1;
+                const int o = io.gd * kF + u;
+                const float x0 = io.cur[u] / IS[o];
+                const float x1 = (io.cur[u] - io.pv[u]) / IS[kOut + o];
+                xs[io.gc * kXS + o] = x0;
+                xs[io.gc * kXS + kOut + o] = x1;
                 ss += x0 * x0 + x1 * x1;
             }
-            red[grp][c] = ss;
+            red[io.gd][io.gc] = ss;
         }
         __syncthreads();
         if (grp == 0) scale[c] = sqrtf((red[0][c] + red[1][c] + red[2][c] + red[3][c]) / (float)kIn);
         __syncthreads();
-        const float rs = scale[c];
-        const float inv = rs > 0.f ? 1.f / rs : 0.f;
-        // 2. h1 = tanh(W1^T (u / s) + b1): outputs 16 grp .. 16 grp + 15
+        // the next tile's band loads overlap this tile's layers
+        if (tile + 1 < t1) fetch_band(nx_io, F, nx, ny, q, tm, prev, tile + 1);
+        // 2-4. The three layers as 16x16x4 f32 matrix-core products: wave wv
+        // owns tile rows (columns) 16 wv .. 16 wv + 15; lane (li, lk) feeds
+        // A[li][lk] = act[16 wv + li][k0 + lk] and B[lk][li] = W[k0 + lk][n0 + li],
+        // and receives D[4 lk + r][li], r = 0..3.  Rows are private to a wave.
+        const int wv = (int)threadIdx.x / 64, ln = (int)threadIdx.x % 64;
+        const int li = ln % 16, lk = ln / 16;
+        const int arow = 16 * wv + li;  // A row of this lane
+        const float ainv = scale[arow] > 0.f ? 1.f / scale[arow] : 0.f;
+        // layer 1: h1 = tanh(W1^T (u / s) + b1), 64 outputs in 4 blocks of 16
         {
-            float acc[16];
+            mf4 acc[4];
 #pragma unroll
-            for (int o = 0; o < 16; ++o) acc[o] = B1[16 * grp + o];
-            for (int in = 0; in < kIn; ++in) {
-                const float xv = xs[c * kXS + in] * inv;
-                const float *wr = W1 + in * kH + 16 * grp;
+            for (int nb = 0; nb < 4; ++nb) acc[nb] = mf4{0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < kIn; k0 += 4) {
+                const float av = xs[arow * kXS + k0 + lk] * ainv;
+                const mf4 bv = *reinterpret_cast<const mf4 *>(W1 + (k0 + lk) * kH + 4 * li);
 #pragma unroll
-                for (int o = 0; o < 16; ++o) acc[o] += xv * wr[o];
+                for (int nb = 0; nb < 4; ++nb)
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[nb], 0, 0, 0);
             }
 #pragma unroll
-            for (int o = 0; o < 16; ++o) hs[c * kHS + 16 * grp + o] = tanhf(acc[o]);
+            for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    hs[(16 * wv + 4 * lk + r) * kHS + 16 * nb + li] = fast_tanh(acc[nb][r] + B1[16 * nb + li]);
         }
         __syncthreads();
-        // 3. h2 = tanh(W2^T h1 + b2), staged in xs
+        // layer 2: h2 = tanh(W2^T h1 + b2), staged in xs
         {
-            float acc[16];
+            mf4 acc[4];
 #pragma unroll
-            for (int o = 0; o < 16; ++o) acc[o] = B2[16 * grp + o];
-            for (int in = 0; in < kH; ++in) {
-                const float hv = hs[c * kHS + in];
-                const float *wr = W2 + in * kH + 16 * grp;
+            for (int nb = 0; nb < 4; ++nb) acc[nb] = mf4{0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < kH; k0 += 4) {
+                const float av = hs[arow * kHS + k0 + lk];
+                const mf4 bv = *reinterpret_cast<const mf4 *>(W2 + (k0 + lk) * kH + 4 * li);
 #pragma unroll
-                for (int o = 0; o < 16; ++o) acc[o] += hv * wr[o];
+                for (int nb = 0; nb < 4; ++nb)
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[nb], 0, 0, 0);
             }
 #pragma unroll
-            for (int o = 0; o < 16; ++o) xs[c * kXS + 16 * grp + o] = tanhf(acc[o]);
+            for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    xs[(16 * wv + 4 * lk + r) * kXS + 16 * nb + li] = fast_tanh(acc[nb][r] + B2[16 * nb + li]);
         }
         __syncthreads();
-        // 4. y = (W3^T h2 + b3) * out_scale * s: outputs 9 grp .. 9 grp + 8,
-        //    i.e. the 9 fields of band depth grp; add to the band, keep it
+        // layer 3: y = (W3^T h2 + b3) * out_scale * s, 36 outputs in 3 blocks
+        // of 16 (the last 12 columns of B are zero), staged in hs
         {
-            float acc[kF];
+            mf4 acc[3];
 #pragma unroll
-            for (int o = 0; o < kF; ++o) acc[o] = B3[kF * grp + o];
-            for (int in = 0; in < kH; ++in) {
-                const float hv = xs[c * kXS + in];
-                const float *wr = W3 + in * kOut + kF * grp;
+            for (int nb = 0; nb < 3; ++nb) acc[nb] = mf4{0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < kH; k0 += 4) {
+                const float av = xs[arow * kXS + k0 + lk];
 #pragma unroll
-                for (int o = 0; o < kF; ++o) acc[o] += hv * wr[o];
-            }
-            if (live) {
-#pragma unroll
-                for (int u = 0; u < kF; ++u) {
-                    const int o = kF * grp + u;
-                    const float y = rs > 0.f ? acc[u] * OS[o] * rs : 0.f;
-                    const float v = F.f[u][pc] + y;
-                    F.f[u][pc] = v;
-                    prev[(long)col * kOut + o] = v;
+                for (int nb = 0; nb < 3; ++nb) {
+                    const int o = 16 * nb + li;
+                    const float bv = o < kOut ? W3[(k0 + lk) * kOut + o] : 0.f;
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[nb], 0, 0, 0);
                 }
             }
+#pragma unroll
+            for (int nb = 0; nb < 3; ++nb) {
+                const int o = 16 * nb + li;
+                if (o < kOut)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * wv + 4 * lk + r;
+                        const float sr = scale[row];
+                        hs[row * kHS + o] = sr > 0.f ? (acc[nb][r] + B3[o]) * OS[o] * sr : 0.f;
+                    }
+            }
         }
+        __syncthreads();
+        // 5. add the correction to band depth gd of column gc, keep the band
+        if (io.live) {
+#pragma unroll
+            for (int u = 0; u < kF; ++u) {
+                const int o = io.gd * kF + u;
+                const float v = io.cur[u] + hs[io.gc * kHS + o];
+                F.f[u][io.pc] = v;
+                prev[(long)io.col * kOut + o] = v;
+            }
+        }
+        io = nx_io;
     }
 }
 
