@@ -1,7 +1,10 @@
 // 3-D staggered-grid velocity/stress sweeps with slab-confined CPML.
 //
-// Reference: src/Elastic3D_SSG_CPML.py:144-309 (update_SSG) and
-// src/Elastic3D_SSG_Tensor.py:166-345 (same maths, other vz term order).
+// Reference: src/Elastic3D_SSG_CPML.py:144-309 (update_SSG),
+// src/Elastic3D_SSG_Tensor.py:166-345 (same maths, other vz term order) and
+// src/Elastic3D_SSG_SPML.py:160-393 (split-field PML: Crank-Nicolson on the
+// damping d of each derivative direction inside the PML, plain updates with
+// 1/rho and point lambda, mu elsewhere).
 //
 // Execution: each thread owns one (i, j) column and marches along z over a
 // chunk of planes; a block takes 256 consecutive columns of a box in x-fastest
@@ -255,6 +258,38 @@ __device__ __forceinline__ Coef coef_of(const sw_axis &ax, int i, bool in_slab)
     return c;
 }
 
+// Split-field PML of one column (src/Elastic3D_SSG_SPML.py): in-PML flags of
+// the integer and half points on x and y (only axes the kernel instantiates;
+// the plan guarantees the others are 0) and the damping there.
+struct SpCol {
+    bool xi, xh, yi, yh;
+    float dxi, dxh, dyi, dyh;
+};
+
+template <int PM>
+__device__ __forceinline__ SpCol spml_col(const sw_ssg3d &g, int i, int j)
+{
+    SpCol c;
+    c.xi = (PM & 1) ? g.px.in[i] != 0 : false;
+    c.xh = (PM & 1) ? g.px.in_h[i] != 0 : false;
+    c.yi = (PM & 2) ? g.py.in[j] != 0 : false;
+    c.yh = (PM & 2) ? g.py.in_h[j] != 0 : false;
+    c.dxi = g.px.d[i];
+    c.dxh = g.px.d_h[i];
+    c.dyi = g.py.d[j];
+    c.dyh = g.py.d_h[j];
+    return c;
+}
+
+// Crank-Nicolson step of one split component: q = (n q + incr) / d
+// (src/Elastic3D_SSG_SPML.py:199-204 and alike); q0 is the value at dst.
+__device__ __forceinline__ float cn_at(float q0, float *dst, float n, float d, float incr)
+{
+    const float v = (n * q0 + incr) / d;
+    *dst = v;
+    return v;
+}
+
 // m = b*m + a*D; D = D/k + m  (src/Elastic3D_SSG_CPML.py:172-177)
 __device__ __forceinline__ float cpml_step(float D, float &m, float a, float b, float k)
 {
@@ -331,7 +366,13 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
     const bool vx_col = i < nx - N - 1 && j >= N + 1;
     const bool vy_col = j < ny - N - 1 && i >= N + 1;
     const bool vz_col = i >= N + 1 && j >= N + 1;
-    const Slab sl = slab_setup<PM>(g, i, j);
+    // CPML axes of this instance (none for the split-field class)
+    constexpr int CPM = CLS == SW_CLASS_SPML ? 0 : PM;
+    constexpr bool SPLIT = CLS == SW_CLASS_SPML && PM != 0;
+    const Slab sl = slab_setup<CPM>(g, i, j);
+    SpCol sp{};
+    if (SPLIT) sp = spml_col<PM>(g, i, j);
+    const long NC = (long)nx * ny * nz;
     const Res R = make_res(g);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
     const Coef cy = coef_of(g.py, j, sl.ys >= 0);
@@ -347,15 +388,28 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
     }
     VPlane<N> P;
 #if SW_PREFETCH
-    load_vplane<N, PM>(P, g, R, pc, k0, SY, SZ, sl);
+    load_vplane<N, CPM>(P, g, R, pc, k0, SY, SZ, sl);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         VPlane<N> Q;
-        if (k + 1 < cl.k1) load_vplane<N, PM>(Q, g, R, pc, k + 1, SY, SZ, sl);
+        if (k + 1 < cl.k1) load_vplane<N, CPM>(Q, g, R, pc, k + 1, SY, SZ, sl);
 #else
-        load_vplane<N, PM>(P, g, R, pc, k, SY, SZ, sl);
+        load_vplane<N, CPM>(P, g, R, pc, k, SY, SZ, sl);
 #endif
+        // split-field PML: flags and damping of this plane, split arrays at (i, j, k)
+        const bool zi = SPLIT && (PM & 4) ? g.pz.in[k] != 0 : false;
+        const bool zh = SPLIT && (PM & 4) ? g.pz.in_h[k] != 0 : false;
+        const float dzi = SPLIT ? g.pz.d[k] : 0.f, dzh = SPLIT ? g.pz.d_h[k] : 0.f;
+        float *qp = SPLIT ? g.split + ((long)k * ny + j) * nx + i : nullptr;
+        const float hdt = 0.5f * dt;
+        // all split values of this cell first, so their loads overlap
+        float qv[9];
+        const bool any_split = SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh);
+        if (any_split) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) qv[t] = qp[(long)t * NC];
+        }
         const uint32_t p = pc + (uint32_t)k * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
         const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
@@ -379,9 +433,17 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
                 dsxz = cpml_step(dsxz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 0 * sl.nzs + oz, P.mz[0]);
             }
-            if (CLS == SW_CLASS_SPML)  // src/Elastic3D_SSG_SPML.py:199-204: x+y+z, times 1/rho
-                st(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * (1.f / P.rho[0]));
-            else
+            if (CLS == SW_CLASS_SPML) {  // src/Elastic3D_SSG_SPML.py:182-205: x+y+z, times 1/rho
+                const float rho_over = 1.f / P.rho[0];
+                if (SPLIT && (sp.xh | sp.yi | zi)) {
+                    const float a = cn_at(qv[0], qp + 0 * NC, 1.f - hdt * sp.dxh, 1.f + hdt * sp.dxh, dsxx * dt * rho_over);
+                    const float b = cn_at(qv[1], qp + 1 * NC, 1.f - hdt * sp.dyi, 1.f + hdt * sp.dyi, dsxy * dt * rho_over);
+                    const float e = cn_at(qv[2], qp + 2 * NC, 1.f - hdt * dzi, 1.f + hdt * dzi, dsxz * dt * rho_over);
+                    st(R.vx, p, a + b + e);
+                } else {
+                    st(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * rho_over);
+                }
+            } else
                 st(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
         }
         // vy at (i, j+1/2, k): :180-195
@@ -402,9 +464,17 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
                 dsyz = cpml_step(dsyz, P.mz[1], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 1 * sl.nzs + oz, P.mz[1]);
             }
-            if (CLS == SW_CLASS_SPML)
-                st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * (1.f / rho_half_y));
-            else
+            if (CLS == SW_CLASS_SPML) {  // :209-233
+                const float rho_over = 1.f / rho_half_y;
+                if (SPLIT && (sp.xi | sp.yh | zi)) {
+                    const float a = cn_at(qv[3], qp + 3 * NC, 1.f - hdt * sp.dxi, 1.f + hdt * sp.dxi, dt * dsxy * rho_over);
+                    const float b = cn_at(qv[4], qp + 4 * NC, 1.f - hdt * sp.dyh, 1.f + hdt * sp.dyh, dt * dsyy * rho_over);
+                    const float e = cn_at(qv[5], qp + 5 * NC, 1.f - hdt * dzi, 1.f + hdt * dzi, dt * dsyz * rho_over);
+                    st(R.vy, p, a + b + e);
+                } else {
+                    st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * rho_over);
+                }
+            } else
                 st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
         }
         // vz at (i, j, k+1/2): :197-212 (Tensor :228-243)
@@ -425,9 +495,17 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
                 dszz = cpml_step(dszz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 2 * sl.nzs + oz, P.mz[2]);
             }
-            if (CLS == SW_CLASS_SPML)
-                st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * (1.f / rho_avg));
-            else if (CLS == SW_CLASS_TENSOR)
+            if (CLS == SW_CLASS_SPML) {  // :235-259
+                const float rho_over = 1.f / rho_avg;
+                if (SPLIT && (sp.xi | sp.yi | zh)) {
+                    const float a = cn_at(qv[6], qp + 6 * NC, 1.f - hdt * sp.dxi, 1.f + hdt * sp.dxi, dt * dsxz * rho_over);
+                    const float b = cn_at(qv[7], qp + 7 * NC, 1.f - hdt * sp.dyi, 1.f + hdt * sp.dyi, dt * dsyz * rho_over);
+                    const float e = cn_at(qv[8], qp + 8 * NC, 1.f - hdt * dzh, 1.f + hdt * dzh, dt * dszz * rho_over);
+                    st(R.vz, p, a + b + e);
+                } else {
+                    st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * rho_over);
+                }
+            } else if (CLS == SW_CLASS_TENSOR)
                 st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt / rho_avg);
             else
                 st(R.vz, p, P.v[2] + (dszz + dsxz + dsyz) * dt / rho_avg);
@@ -516,7 +594,12 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
     const bool sxy_col = i >= N + 1 && j >= N + 1;
     const bool sxz_col = i < nx - N - 1;
     const bool syz_col = j < ny - N - 1;
-    const Slab sl = slab_setup<PM>(g, i, j);
+    constexpr int CPM = CLS == SW_CLASS_SPML ? 0 : PM;
+    constexpr bool SPLIT = CLS == SW_CLASS_SPML && PM != 0;
+    const Slab sl = slab_setup<CPM>(g, i, j);
+    SpCol sp{};
+    if (SPLIT) sp = spml_col<PM>(g, i, j);
+    const long NC = (long)nx * ny * nz;
     const Res R = make_res(g);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
     const Coef cy = coef_of(g.py, j, sl.ys >= 0);
@@ -532,15 +615,28 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
     }
     SPlane<N> P;
 #if SW_PREFETCH
-    load_splane<N, PM>(P, g, R, pc, k0, SY, SZ, sl);
+    load_splane<N, CPM>(P, g, R, pc, k0, SY, SZ, sl);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         SPlane<N> Q;
-        if (k + 1 < cl.k1) load_splane<N, PM>(Q, g, R, pc, k + 1, SY, SZ, sl);
+        if (k + 1 < cl.k1) load_splane<N, CPM>(Q, g, R, pc, k + 1, SY, SZ, sl);
 #else
-        load_splane<N, PM>(P, g, R, pc, k, SY, SZ, sl);
+        load_splane<N, CPM>(P, g, R, pc, k, SY, SZ, sl);
 #endif
+        const bool zi = SPLIT && (PM & 4) ? g.pz.in[k] != 0 : false;
+        const bool zh = SPLIT && (PM & 4) ? g.pz.in_h[k] != 0 : false;
+        const float dzi = SPLIT ? g.pz.d[k] : 0.f, dzh = SPLIT ? g.pz.d_h[k] : 0.f;
+        float *qp = SPLIT ? g.split + ((long)k * ny + j) * nx + i : nullptr;
+        const float hdt = 0.5f * dt;
+        // all split values of this cell first (components 9..23), so their
+        // loads overlap
+        float qv[15];
+        const bool any_split = SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh);
+        if (any_split) {
+#pragma unroll
+            for (int t = 0; t < 15; ++t) qv[t] = qp[(long)(9 + t) * NC];
+        }
         const uint32_t p = pc + (uint32_t)k * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
         const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
@@ -569,9 +665,26 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
                 dvzz = cpml_step(dvzz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 3 * sl.nzs + oz, P.mz[0]);
             }
-            st(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
-            st(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
-            st(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
+            if (SPLIT && (sp.xi | sp.yi | zi)) {  // src/Elastic3D_SSG_SPML.py:296-333
+                const float xn = 1.f - hdt * sp.dxi, yn = 1.f - hdt * sp.dyi, zn = 1.f - hdt * dzi;
+                const float xd = 1.f + hdt * sp.dxi, yd = 1.f + hdt * sp.dyi, zd = 1.f + hdt * dzi;
+                const float a0 = cn_at(qv[0], qp + 9 * NC, xn, xd, dt * lpm * dvxx);
+                const float a1 = cn_at(qv[1], qp + 10 * NC, yn, yd, dt * lam_h * dvyy);
+                const float a2 = cn_at(qv[2], qp + 11 * NC, zn, zd, dt * lam_h * dvzz);
+                const float b0 = cn_at(qv[3], qp + 12 * NC, xn, xd, dt * lam_h * dvxx);
+                const float b1 = cn_at(qv[4], qp + 13 * NC, yn, yd, dt * lpm * dvyy);
+                const float b2 = cn_at(qv[5], qp + 14 * NC, zn, zd, dt * lam_h * dvzz);
+                const float c0 = cn_at(qv[12], qp + 21 * NC, xn, xd, dt * lam_h * dvxx);
+                const float c1 = cn_at(qv[13], qp + 22 * NC, yn, yd, dt * lam_h * dvyy);
+                const float c2 = cn_at(qv[14], qp + 23 * NC, zn, zd, dt * lpm * dvzz);
+                st(R.sxx, p, a0 + a1 + a2);
+                st(R.syy, p, b0 + b1 + b2);
+                st(R.szz, p, c0 + c1 + c2);
+            } else {
+                st(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
+                st(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
+                st(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
+            }
         }
         // sxy at (i+1/2, j+1/2, k): :267-279
         if (sxy_col) {
@@ -586,7 +699,13 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
                 dvxy = cpml_step(dvxy, P.my[1], cy.ah, cy.bh, cy.kh);
                 st(R.my, 4 * sl.nys + oy, P.my[1]);
             }
-            st(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
+            if (SPLIT && (sp.xh | sp.yh | zi)) {  // :334-350
+                const float a = cn_at(qv[6], qp + 15 * NC, 1.f - hdt * sp.dxh, 1.f + hdt * sp.dxh, dt * mu_a * dvyx);
+                const float b = cn_at(qv[7], qp + 16 * NC, 1.f - hdt * sp.dyh, 1.f + hdt * sp.dyh, dt * mu_a * dvxy);
+                st(R.sxy, p, a + b);
+            } else {
+                st(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
+            }
         }
         // sxz at (i+1/2, j, k+1/2): :281-293
         if (sxz_col && k < nz - N - 1) {
@@ -601,7 +720,13 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
                 dvxz = cpml_step(dvxz, P.mz[1], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 4 * sl.nzs + oz, P.mz[1]);
             }
-            st(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
+            if (SPLIT && (sp.xh | sp.yi | zh)) {  // :352-371
+                const float a = cn_at(qv[8], qp + 17 * NC, 1.f - hdt * sp.dxh, 1.f + hdt * sp.dxh, dt * mu_a * dvzx);
+                const float b = cn_at(qv[9], qp + 18 * NC, 1.f - hdt * dzh, 1.f + hdt * dzh, dt * mu_a * dvxz);
+                st(R.sxz, p, a + b);
+            } else {
+                st(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
+            }
         }
         // syz at (i, j+1/2, k+1/2): :295-307
         if (syz_col && k < nz - N - 1) {
@@ -616,7 +741,13 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
                 dvyz = cpml_step(dvyz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 5 * sl.nzs + oz, P.mz[2]);
             }
-            st(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
+            if (SPLIT && (sp.xi | sp.yh | zh)) {  // :373-391
+                const float a = cn_at(qv[10], qp + 19 * NC, 1.f - hdt * sp.dyh, 1.f + hdt * sp.dyh, dt * mu_a * dvzy);
+                const float b = cn_at(qv[11], qp + 20 * NC, 1.f - hdt * dzh, 1.f + hdt * dzh, dt * mu_a * dvyz);
+                st(R.syz, p, a + b);
+            } else {
+                st(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
+            }
         }
         win_shift<N>(wvz);
         win_shift<N>(wvx);
@@ -704,39 +835,21 @@ void launch_velocity(const sw_ssg3d &g, const Plan &p, hipStream_t st)
     }
 }
 
-template <int N>
+template <int N, int CLS>
 void launch_stress(const sw_ssg3d &g, const Plan &p, hipStream_t st)
 {
     for (int t = 0; t < p.n; ++t) {
         const dim3 grd(p.g[t].nblock), blk(kThreads);
         if (p.pm[t] == 1)
-            hipLaunchKernelGGL((k_stress<N, SW_CLASS_CPML, 1>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, CLS, 1>), grd, blk, 0, st, g, p.g[t]);
         else if (p.pm[t] == 3)
-            hipLaunchKernelGGL((k_stress<N, SW_CLASS_CPML, 3>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, CLS, 3>), grd, blk, 0, st, g, p.g[t]);
         else
-            hipLaunchKernelGGL((k_stress<N, SW_CLASS_CPML, 7>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, CLS, 7>), grd, blk, 0, st, g, p.g[t]);
     }
 }
 
 }  // namespace
-
-// Split-field PML class, split-free interior box only: the plain updates of
-// src/Elastic3D_SSG_SPML.py (no CPML code instantiated, PM = 0).
-int launch_ssg3d_spml_interior(const sw_ssg3d *g, int i0, int i1, int j0, int j1, int k0, int k1, bool stress,
-                               hipStream_t st)
-{
-    Plan p;
-    add_box(p, 0, i0, i1, j0, j1, k0, k1);
-    if (p.n == 0) return SW_OK;
-    const dim3 grd(p.g[0].nblock), blk(kThreads);
-    SW_DISPATCH_ORDER(g->order, {
-        if (stress)
-            hipLaunchKernelGGL((k_stress<N, SW_CLASS_SPML, 0>), grd, blk, 0, st, *g, p.g[0]);
-        else
-            hipLaunchKernelGGL((k_velocity<N, SW_CLASS_SPML, 0>), grd, blk, 0, st, *g, p.g[0]);
-    });
-    return SW_OK;
-}
 
 int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st)
 {
@@ -752,8 +865,22 @@ int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st)
 
 int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st)
 {
-    SW_DISPATCH_ORDER(g->order, { launch_stress<N>(*g, make_plan(*g, N), st); });
+    SW_DISPATCH_ORDER(g->order, { launch_stress<N, SW_CLASS_CPML>(*g, make_plan(*g, N), st); });
     return launch_status("ssg3d stress");
+}
+
+// Split-field PML class (src/Elastic3D_SSG_SPML.py:160-393): the same plan,
+// the axes' lo/hi bounding their split-free ranges.
+int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st)
+{
+    SW_DISPATCH_ORDER(g->order, { launch_velocity<N, SW_CLASS_SPML>(*g, make_plan(*g, N), st); });
+    return launch_status("ssg3d velocity (split-field PML)");
+}
+
+int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st)
+{
+    SW_DISPATCH_ORDER(g->order, { launch_stress<N, SW_CLASS_SPML>(*g, make_plan(*g, N), st); });
+    return launch_status("ssg3d stress (split-field PML)");
 }
 
 }  // namespace sw
