@@ -215,6 +215,30 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
 int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_adj2d *adj,
                              const sw_receivers *residual, float *work, int32_t it0, int32_t n, void *stream);
 
+/* ---- shot batches (no counterpart in the reference): nbatch shots of the
+ * SW_2D_VS scheme in each launch.  Shot b's state is the 15-plane block at
+ * g->vx + b*state_stride (layout as for sw_ssg2d_adjoint_segment), its
+ * source is src (explosive or moment tensor, shared wavelet) at cell
+ * (cells[2b], cells[2b+1]), its traces / residuals are rec->data +
+ * b*data_stride, and its adjoint block is every sw_adj2d pointer +
+ * b*adj_stride (24 planes: 15 state, 4 scratch, 5 accumulators).  The
+ * model, PML and receiver positions are shared. */
+#define SW_MAX_BATCH 64
+/* nsteps of sw_ssg2d_run (stress source before the velocity sweep) for every shot */
+int sw_ssg2d_run_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
+                       const int32_t *cells, const sw_receivers *rec, int64_t data_stride, int32_t it0,
+                       int32_t nsteps, void *stream);
+/* sw_ssg2d_adjoint_segment for every shot; work holds 2*n*nbatch*state_stride floats
+ * (2*n*15*nx*nz for one shot) */
+int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
+                                   const int32_t *cells, const sw_adj2d *adj, int64_t adj_stride,
+                                   const sw_receivers *residual, int64_t data_stride, float *work, int32_t it0,
+                                   int32_t n, void *stream);
+/* sw_ssg2d_model_gradient of the accumulators summed over the shots */
+int sw_ssg2d_model_gradient_batch(const sw_ssg2d *g, const sw_adj2d *adj, int32_t nbatch, int64_t adj_stride,
+                                  const float *vp, const float *vs, float *gvp, float *gvs, float *grho,
+                                  void *stream);
+
 /* ---- learned thin boundary (no counterpart in the reference: a thin CPML
  * plus a per-boundary-column MLP in place of most of a wide CPML) ----
  * The interior box (cells outside every CPML slab) starts at (ox, oy, oz)

@@ -96,6 +96,11 @@ SIGNATURES = {
     "sw_ssg2d_adjoint_inject": [_P2, _PA, _PR, C.c_int32, C.c_void_p],
     "sw_ssg2d_model_gradient": [_P2, _PA, _fp, _fp, _fp, _fp, _fp, C.c_void_p],
     "sw_ssg2d_adjoint_segment": [_P2, _PS, _PA, _PR, _fp, C.c_int32, C.c_int32, C.c_void_p],
+    "sw_ssg2d_run_batch": [_P2, C.c_int32, C.c_int64, _PS, C.POINTER(C.c_int32), _PR, C.c_int64, C.c_int32,
+                           C.c_int32, C.c_void_p],
+    "sw_ssg2d_adjoint_segment_batch": [_P2, C.c_int32, C.c_int64, _PS, C.POINTER(C.c_int32), _PA, C.c_int64, _PR,
+                                       C.c_int64, _fp, C.c_int32, C.c_int32, C.c_void_p],
+    "sw_ssg2d_model_gradient_batch": [_P2, _PA, C.c_int32, C.c_int64, _fp, _fp, _fp, _fp, _fp, C.c_void_p],
     "sw_mlp_param_count": [],
     "sw_band_columns": [_PB],
     "sw_band_gather": [_P3, _PB, _fp, C.c_void_p],

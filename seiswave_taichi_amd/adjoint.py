@@ -30,6 +30,7 @@ from .coeffs import diff_coff, ricker2
 from .pml import axis_pml, check_parameter
 
 NSTATE = 15  # vx vz sxx szz sxz + 10 memory variables
+MAX_BATCH = 64  # SW_MAX_BATCH
 _FIELD_ROW = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
 
 
@@ -39,12 +40,14 @@ class Elastic2DModeling:
     shape (nx, nz); shots: list of (ix, iz) source cells; receivers: (N, 2)
     cells; components: recorded fields.  ``source`` is "explosive" or a
     moment tensor (Mxx, Mzz, Mxz).  ``wavelet`` defaults to a Ricker of
-    ``freq`` delayed by 1.5/freq.
+    ``freq`` delayed by 1.5/freq.  With ``batch`` (the default) all shots (up
+    to 64 at a time) advance together in each launch (sw_ssg2d_*_batch);
+    otherwise shot by shot.
     """
 
     def __init__(self, shape, dx, dz, dt, nt, accuracy, pml: dict, shots, receivers,
                  pml_surface=(True, True, True, True), freq=10.0, components=("vx", "vz"),
-                 source="explosive", wavelet=None, checkpoint=50):
+                 source="explosive", wavelet=None, checkpoint=50, batch=True):
         check_parameter(pml, ("pml_x_thick", "pml_z_thick"))
         self.lib = L.load()
         self.device = gpu_device()
@@ -52,6 +55,7 @@ class Elastic2DModeling:
         self.dx, self.dz, self.dt, self.nt, self.order = dx, dz, dt, int(nt), int(accuracy)
         self.c = diff_coff(self.order)
         self.checkpoint = int(checkpoint)
+        self.batch = bool(batch)
         dev = self.device
         ax = axis_pml(self.nx, self.order, dx, dt, pml["pml_x_thick"], pml["vp_max"], pml["Rcoef"],
                       pml["alpha_max_pml"], pml["kmax_pml"], pml["theta"], pml_surface[0], pml_surface[1])
@@ -160,8 +164,20 @@ class Elastic2DModeling:
             L.check(lib.sw_ssg2d_run(g, src, rec, s0, n, st), "sw_ssg2d_run")
         return state
 
+    def _groups(self):
+        n = len(self.shots)
+        return [list(range(a, min(a + MAX_BATCH, n))) for a in range(0, n, MAX_BATCH)]
+
+    def _cells(self, group):
+        arr = (C.c_int32 * (2 * len(group)))()
+        for t, s in enumerate(group):
+            arr[2 * t], arr[2 * t + 1] = self.shots[s]
+        return arr
+
     def model(self, vp, vs, rho, keep_checkpoints=False):
         """Seismograms (nshot, nt, ncomp, nrec) and, optionally, the checkpoints."""
+        if self.batch:
+            return self._model_batch(vp, vs, rho, keep_checkpoints)
         self.set_model(vp, vs, rho)
         nck = (self.nt + self.checkpoint - 1) // self.checkpoint
         data = torch.zeros((len(self.shots), self.nt, len(self.components), self.nrec), dtype=torch.float32,
@@ -173,10 +189,70 @@ class Elastic2DModeling:
             cks.append(ck)
         return data, cks
 
+    def _model_batch(self, vp, vs, rho, keep_checkpoints):
+        self.set_model(vp, vs, rho)
+        lib, st = self.lib, self.stream()
+        K, nt = self.checkpoint, self.nt
+        nck = (nt + K - 1) // K
+        ncomp = len(self.components)
+        data = torch.zeros((len(self.shots), nt, ncomp, self.nrec), dtype=torch.float32, device=self.device)
+        cks = []
+        for group in self._groups():
+            state = self.new_state(len(group))
+            g = self.desc(state[0])
+            src = self.source(self.shots[group[0]])
+            rec = self.receivers(data[group[0]])
+            cells = self._cells(group)
+            ck = self.new_state(nck * len(group)).view(nck, len(group), NSTATE, self.nz, self.nx) \
+                if keep_checkpoints else None
+            for s0 in range(0, nt, K):
+                if ck is not None:
+                    ck[s0 // K].copy_(state)
+                n = min(K, nt - s0)
+                L.check(lib.sw_ssg2d_run_batch(g, len(group), NSTATE * self.nx * self.nz, src, cells, rec,
+                                               nt * ncomp * self.nrec, s0, n, st), "sw_ssg2d_run_batch")
+            cks.append(ck)
+        return data, cks
+
+    def _gradient_batch(self, grad_data, checkpoints):
+        lib, st = self.lib, self.stream()
+        nz, nx, K = self.nz, self.nx, self.checkpoint
+        P = nx * nz
+        ncomp = len(self.components)
+        out = torch.empty((3, nz, nx), dtype=torch.float32, device=self.device)
+        total = None
+        for group, ck in zip(self._groups(), checkpoints):
+            nb = len(group)
+            # per shot: 15 adjoint state planes, 4 scratch, 5 accumulators
+            adjb = torch.zeros((nb, 24, nz, nx), dtype=torch.float32, device=self.device)
+            q = self._adj_desc(adjb[0, :NSTATE], adjb[0, NSTATE:NSTATE + 4], adjb[0, NSTATE + 4:])
+            work = self.new_state(nb)
+            seg = self.new_state(2 * K * nb)
+            g = self.desc(work[0])
+            src = self.source(self.shots[group[0]])
+            res = grad_data[group[0]:group[-1] + 1].detach().to(torch.float32).contiguous()
+            rres = self.receivers(res[0])
+            cells = self._cells(group)
+            for s0 in reversed(range(0, self.nt, K)):
+                n = min(K, self.nt - s0)
+                work.copy_(ck[s0 // K])
+                L.check(lib.sw_ssg2d_adjoint_segment_batch(g, nb, NSTATE * P, src, cells, q, 24 * P, rres,
+                                                           self.nt * ncomp * self.nrec, L.fptr(seg), s0, n, st),
+                        "sw_ssg2d_adjoint_segment_batch")
+            part = torch.empty((3, nz, nx), dtype=torch.float32, device=self.device)
+            L.check(lib.sw_ssg2d_model_gradient_batch(g, q, nb, 24 * P, L.fptr(self.vp_d), L.fptr(self.vs_d),
+                                                      L.fptr(part[0]), L.fptr(part[1]), L.fptr(part[2]), st),
+                    "sw_ssg2d_model_gradient_batch")
+            total = part if total is None else total + part
+        out.copy_(total)
+        return tuple(out[i].t() for i in range(3))
+
     # -------------------------------------------------------------- backward
     def gradient(self, grad_data: torch.Tensor, checkpoints):
         """d loss / d (vp, vs, rho) for d loss / d data = grad_data, as (nx, nz)
         tensors in reference order.  Requires the model set by ``model``."""
+        if self.batch:
+            return self._gradient_batch(grad_data, checkpoints)
         lib, st = self.lib, self.stream()
         nz, nx, K = self.nz, self.nx, self.checkpoint
         acc = torch.zeros((5, nz, nx), dtype=torch.float32, device=self.device)  # g_b1 g_b4 g_lamh g_muh g_muz

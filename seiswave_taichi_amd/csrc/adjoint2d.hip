@@ -37,8 +37,10 @@ __device__ __forceinline__ float fwd_cpml(float D, float m, float a, float b, fl
 
 // ------------------------------------------------------------- stress stage
 template <int N>
-__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q)
+__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, long fs, long as)
 {
+    shift_state(f, (long)blockIdx.z * fs);
+    shift_adj(q, (long)blockIdx.z * as);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
@@ -110,8 +112,10 @@ __device__ __forceinline__ float tap(const float *t, long idx, long n)
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q)
+__global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q, long fs, long as)
 {
+    shift_state(f, (long)blockIdx.z * fs);
+    shift_adj(q, (long)blockIdx.z * as);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
@@ -138,8 +142,10 @@ __global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q)
 
 // ----------------------------------------------------------- velocity stage
 template <int N>
-__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q)
+__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, long fs, long as)
 {
+    shift_state(f, (long)blockIdx.z * fs);
+    shift_adj(q, (long)blockIdx.z * as);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
@@ -200,8 +206,10 @@ __global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q)
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q)
+__global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q, long fs, long as)
 {
+    shift_state(f, (long)blockIdx.z * fs);
+    shift_adj(q, (long)blockIdx.z * as);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
@@ -230,10 +238,13 @@ __global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q)
 // Residual injection: adjoint of the receiver gather data[it] = field[idx].
 // One thread per (component, receiver).  Receivers at the same cell add up;
 // with such duplicates their sum order is not fixed (hardware f32 atomics).
-__global__ void k_adj_inject(sw_adj2d q, sw_receivers r, int it)
+// Shot blockIdx.y of a batch: adjoint at + y * as, residuals at + y * ds.
+__global__ void k_adj_inject(sw_adj2d q, sw_receivers r, int it, long as, long ds)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= r.ncomp * r.nrec) return;
+    shift_adj(q, (long)blockIdx.y * as);
+    r.data += (long)blockIdx.y * ds;
     const int comp = t / r.nrec, k = t % r.nrec;
     float *fld = r.comp[comp] == 0 ? q.vx : (r.comp[comp] == 2 ? q.vz : (r.comp[comp] == 3 ? q.sxx :
                  (r.comp[comp] == 5 ? q.szz : q.sxz)));
@@ -241,12 +252,21 @@ __global__ void k_adj_inject(sw_adj2d q, sw_receivers r, int it)
     atomicAdd(fld + r.idx[k], res);
 }
 
-// Model gradient from the accumulated coefficient gradients:
+// Sum over the nb shots of a batch of one accumulator at cell p.
+__device__ __forceinline__ float acc_sum(const float *a, long p, int nb, long as)
+{
+    float v = a[p];
+    for (int b = 1; b < nb; ++b) v += a[p + (long)b * as];
+    return v;
+}
+
+// Model gradient from the accumulated coefficient gradients (summed over the
+// nb shots of a batch first):
 //   lam_h = (lam[p]+lam[p+1])/2, mu_h likewise (sxx cells); mu_z = (mu[p+SZ]+mu[p])/2
 //   (sxz cells); b1 = dt/rho (vx cells); b4 = dt/rho4 (vz cells); then
 //   mu = rho vs^2, lam = rho (vp^2 - 2 vs^2).
 __global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const float *vs, float *gvp,
-                                 float *gvs, float *grho)
+                                 float *gvs, float *grho, int nb, long as)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -257,11 +277,11 @@ __global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const 
     const float dt = f.dt;
     // gl = d/d lam; gb = d/d mu - 2 d/d lam (g_muh holds the sxx-cell part of the
     // latter, g_muz the sxz-cell mu gradient)
-    float gl = 0.5f * q.g_lamh[p] + (i >= 1 ? 0.5f * q.g_lamh[p - 1] : 0.f);
-    float gb = 0.5f * q.g_muh[p] + (i >= 1 ? 0.5f * q.g_muh[p - 1] : 0.f);
-    gb += 0.5f * q.g_muz[p] + (j >= 1 ? 0.5f * q.g_muz[p - SZ] : 0.f);
+    float gl = 0.5f * acc_sum(q.g_lamh, p, nb, as) + (i >= 1 ? 0.5f * acc_sum(q.g_lamh, p - 1, nb, as) : 0.f);
+    float gb = 0.5f * acc_sum(q.g_muh, p, nb, as) + (i >= 1 ? 0.5f * acc_sum(q.g_muh, p - 1, nb, as) : 0.f);
+    gb += 0.5f * acc_sum(q.g_muz, p, nb, as) + (j >= 1 ? 0.5f * acc_sum(q.g_muz, p - SZ, nb, as) : 0.f);
     const float r = f.rho[p];
-    float gr = q.g_b1[p] * (-dt / (r * r));
+    float gr = acc_sum(q.g_b1, p, nb, as) * (-dt / (r * r));
     // vz cell p' uses rho4 over (p', p'+SZ, p'+1+SZ, p'+1): cell p is in the stencils
     // of p, p-SZ, p-1-SZ, p-1
     for (int a = 0; a < 2; ++a)
@@ -270,7 +290,7 @@ __global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const 
             if (ii < 0 || jj < 0 || ii >= nx - 1 || jj >= nz - 1) continue;
             const long pp = (long)jj * SZ + ii;
             const float r4 = 0.25f * (f.rho[pp] + f.rho[pp + SZ] + f.rho[pp + 1 + SZ] + f.rho[pp + 1]);
-            gr += 0.25f * q.g_b4[pp] * (-dt / (r4 * r4));
+            gr += 0.25f * acc_sum(q.g_b4, pp, nb, as) * (-dt / (r4 * r4));
         }
     // lam = rho (vp^2 - 2 vs^2), mu = rho vs^2:
     //   d/dvp = 2 rho vp lam_bar;  d/dvs = 2 rho vs (mu_bar - 2 lam_bar);
@@ -281,46 +301,69 @@ __global__ void k_model_gradient(sw_ssg2d f, sw_adj2d q, const float *vp, const 
     grho[p] = gr + gl * (a * a) + gb * (s * s);
 }
 
-inline dim3 grid2(const sw_ssg2d &g, dim3 blk)
+inline dim3 grid2(const sw_ssg2d &g, dim3 blk, int nbatch = 1)
 {
-    return dim3((g.nx + blk.x - 1) / blk.x, (g.nz + blk.y - 1) / blk.y, 1);
+    return dim3((g.nx + blk.x - 1) / blk.x, (g.nz + blk.y - 1) / blk.y, nbatch);
 }
 
 }  // namespace
 
-int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
+int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(f->order, {
-        hipLaunchKernelGGL((k_adj_stress_a<N>), grid2(*f, blk), blk, 0, st, *f, *q);
-        hipLaunchKernelGGL((k_adj_stress_b<N>), grid2(*f, blk), blk, 0, st, *f, *q);
+        hipLaunchKernelGGL((k_adj_stress_a<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
+        hipLaunchKernelGGL((k_adj_stress_b<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
     });
     return launch_status("2-D adjoint stress stage");
 }
 
-int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
+int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as,
+                                hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(f->order, {
-        hipLaunchKernelGGL((k_adj_velocity_a<N>), grid2(*f, blk), blk, 0, st, *f, *q);
-        hipLaunchKernelGGL((k_adj_velocity_b<N>), grid2(*f, blk), blk, 0, st, *f, *q);
+        hipLaunchKernelGGL((k_adj_velocity_a<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
+        hipLaunchKernelGGL((k_adj_velocity_b<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
     });
     return launch_status("2-D adjoint velocity stage");
 }
 
-int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st)
+int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
+{
+    return launch_adj2d_stress_batch(f, q, 1, 0, 0, st);
+}
+
+int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
+{
+    return launch_adj2d_velocity_batch(f, q, 1, 0, 0, st);
+}
+
+int launch_adj2d_inject_batch(const sw_adj2d *q, const sw_receivers *r, int it, int nbatch, long as, long ds,
+                              hipStream_t st)
 {
     const int n = r->ncomp * r->nrec;
-    hipLaunchKernelGGL(k_adj_inject, dim3((n + 255) / 256), dim3(256), 0, st, *q, *r, it);
+    hipLaunchKernelGGL(k_adj_inject, dim3((n + 255) / 256, nbatch), dim3(256), 0, st, *q, *r, it, as, ds);
     return launch_status("2-D adjoint residual injection");
+}
+
+int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st)
+{
+    return launch_adj2d_inject_batch(q, r, it, 1, 0, 0, st);
+}
+
+int launch_adj2d_model_gradient_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, const float *vp,
+                                      const float *vs, float *gvp, float *gvs, float *grho, hipStream_t st)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_model_gradient, grid2(*f, blk), blk, 0, st, *f, *q, vp, vs, gvp, gvs, grho, nbatch, as);
+    return launch_status("2-D model gradient");
 }
 
 int launch_adj2d_model_gradient(const sw_ssg2d *f, const sw_adj2d *q, const float *vp, const float *vs, float *gvp,
                                 float *gvs, float *grho, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_model_gradient, grid2(*f, blk), blk, 0, st, *f, *q, vp, vs, gvp, gvs, grho);
-    return launch_status("2-D model gradient");
+    return launch_adj2d_model_gradient_batch(f, q, 1, 0, vp, vs, gvp, gvs, grho, st);
 }
 
 }  // namespace sw

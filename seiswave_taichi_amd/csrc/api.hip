@@ -35,6 +35,15 @@ int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, f
                 hipStream_t st);
 int mlp_param_count();
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
+int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st);
+int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st);
+int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, hipStream_t st);
+int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as,
+                                hipStream_t st);
+int launch_adj2d_inject_batch(const sw_adj2d *q, const sw_receivers *r, int it, int nbatch, long as, long ds,
+                              hipStream_t st);
+int launch_adj2d_model_gradient_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, const float *vp,
+                                      const float *vs, float *gvp, float *gvs, float *grho, hipStream_t st);
 int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStream_t st);
 int launch_adj2d_model_gradient(const sw_ssg2d *f, const sw_adj2d *q, const float *vp, const float *vs, float *gvp,
                                 float *gvs, float *grho, hipStream_t st);
@@ -121,7 +130,7 @@ __global__ void k_inject3d(sw_ssg3d g, sw_source s, int it)
     }
 }
 
-__global__ void k_inject2d(sw_ssg2d g, sw_source s, int it)
+__device__ __forceinline__ void inject2d(sw_ssg2d &g, const sw_source &s, int it)
 {
     const float amp = s.amp[it];
     const long p = (long)s.iz * g.nx + s.ix;
@@ -149,6 +158,38 @@ __global__ void k_inject2d(sw_ssg2d g, sw_source s, int it)
     } break;
     default: break;
     }
+}
+
+__global__ void k_inject2d(sw_ssg2d g, sw_source s, int it)
+{
+    inject2d(g, s, it);
+}
+
+// Source cells of the shots of a batch (at most SW_MAX_BATCH).
+struct ShotCells {
+    int ix[SW_MAX_BATCH], iz[SW_MAX_BATCH];
+};
+
+// One block per shot: shot b's state at + b * bs, its source at cell b.
+__global__ void k_inject2d_batch(sw_ssg2d g, sw_source s, ShotCells sc, int it, long bs)
+{
+    const int b = blockIdx.x;
+    shift_state(g, (long)b * bs);
+    s.ix = sc.ix[b];
+    s.iz = sc.iz[b];
+    inject2d(g, s, it);
+}
+
+// Receivers of shot blockIdx.y: state at + y * bs, traces at + y * ds.
+__global__ void k_record2d_batch(sw_ssg2d g, sw_receivers r, int it, long bs, long ds)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= r.nrec * r.ncomp) return;
+    shift_state(g, (long)blockIdx.y * bs);
+    const int comp = t / r.nrec, rr = t - comp * r.nrec;
+    const int code = r.comp[comp];
+    const float *f = code == 0 ? g.vx : code == 2 ? g.vz : code == 3 ? g.sxx : code == 5 ? g.szz : g.sxz;
+    r.data[(long)blockIdx.y * ds + ((long)it * r.ncomp + comp) * r.nrec + rr] = f[r.idx[rr]];
 }
 
 // ---------------------------------------------------------------- receivers
@@ -521,6 +562,130 @@ int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_a
         if ((rc = launch_adj2d_velocity(&pre, adj, st))) return rc;
     }
     return launch_status("ssg2d adjoint segment");
+}
+
+// ---- shot batches of the 2-D VS path ----
+static int check_batch2d(const sw_ssg2d *g, int nbatch, long state_stride, const sw_source *src,
+                         const int32_t *cells, ShotCells &sc)
+{
+    int rc = check_ssg2d(g);
+    if (rc) return rc;
+    SW_REQUIRE(nbatch >= 1 && nbatch <= SW_MAX_BATCH, "batch of %d shots outside 1..%d", nbatch, SW_MAX_BATCH);
+    SW_REQUIRE(src && cells, "null source or shot cells");
+    const long S = 15L * g->nx * g->nz;
+    SW_REQUIRE(nbatch == 1 || state_stride >= S, "state stride %ld below one state (%ld floats)", state_stride, S);
+    const long P = (long)g->nx * g->nz;
+    SW_REQUIRE(g->vz == g->vx + P && g->sxx == g->vx + 2 * P && g->szz == g->vx + 3 * P &&
+                   g->sxz == g->vx + 4 * P && g->mem == g->vx + 5 * P,
+               "the state must be one block of 15 planes (vx vz sxx szz sxz mem) starting at vx");
+    SW_REQUIRE(src->kind == SW_SRC_NONE || src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE,
+               "shot batches take stress sources (explosive or moment tensor)");
+    for (int b = 0; b < nbatch; ++b) {
+        const int ix = cells[2 * b], iz = cells[2 * b + 1];
+        SW_REQUIRE(ix >= 1 && ix < g->nx - 1 && iz >= 1 && iz < g->nz - 1, "shot %d cell (%d,%d) outside the grid", b,
+                   ix, iz);
+        sc.ix[b] = ix;
+        sc.iz[b] = iz;
+    }
+    return SW_OK;
+}
+
+static int check_batch_receivers(const sw_ssg2d *g, const sw_receivers *r, int it0, int n)
+{
+    int rc;
+    if ((rc = check_receivers(r, it0)) || (rc = check_receivers(r, it0 + n - 1))) return rc;
+    SW_REQUIRE(r->ncomp <= 16, "too many components");
+    for (int c = 0; c < r->ncomp; ++c)
+        SW_REQUIRE(fields2d(g).f[r->comp[c]] != nullptr, "component %d does not exist in 2-D", r->comp[c]);
+    return SW_OK;
+}
+
+int sw_ssg2d_run_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
+                       const int32_t *cells, const sw_receivers *rec, int64_t data_stride, int32_t it0,
+                       int32_t nsteps, void *stream)
+{
+    ShotCells sc{};
+    int rc = check_batch2d(g, nbatch, state_stride, src, cells, sc);
+    if (rc) return rc;
+    SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");
+    if (nsteps == 0) return SW_OK;
+    if (src->kind != SW_SRC_NONE) {
+        SW_REQUIRE(src->amp != nullptr && it0 + nsteps <= src->namp, "source amplitudes missing for the steps");
+    }
+    if (rec && (rc = check_batch_receivers(g, rec, it0, nsteps))) return rc;
+    hipStream_t st = as_stream(stream);
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        if (src->kind != SW_SRC_NONE)
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, (long)state_stride);
+        if ((rc = launch_ssg2d_velocity_batch(g, nbatch, state_stride, st))) return rc;
+        if ((rc = launch_ssg2d_stress_batch(g, nbatch, state_stride, st))) return rc;
+        if (rec && rec->nrec) {
+            const int n = rec->nrec * rec->ncomp;
+            hipLaunchKernelGGL(k_record2d_batch, dim3((n + 255) / 256, nbatch), dim3(256), 0, st, *g, *rec, it,
+                               (long)state_stride, (long)data_stride);
+        }
+    }
+    return launch_status("ssg2d batch run");
+}
+
+int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
+                                   const int32_t *cells, const sw_adj2d *adj, int64_t adj_stride,
+                                   const sw_receivers *residual, int64_t data_stride, float *work, int32_t it0,
+                                   int32_t n, void *stream)
+{
+    ShotCells sc{};
+    int rc = check_batch2d(g, nbatch, state_stride, src, cells, sc);
+    if (rc) return rc;
+    if ((rc = check_adj2d(g, adj))) return rc;
+    SW_REQUIRE(residual && work, "null residual or work buffer");
+    SW_REQUIRE(n >= 0 && it0 >= 0, "bad step range");
+    if (n == 0) return SW_OK;
+    if (src->kind != SW_SRC_NONE) {
+        SW_REQUIRE(src->amp != nullptr && it0 + n <= src->namp, "source amplitudes missing for the steps");
+    }
+    if ((rc = check_batch_receivers(g, residual, it0, n))) return rc;
+    const long P = (long)g->nx * g->nz, S = 15 * P;
+    SW_REQUIRE(nbatch == 1 || adj_stride >= 24 * P, "adjoint stride %ld below one adjoint block (%ld floats)",
+               (long)adj_stride, 24 * P);
+    const long bs = nbatch == 1 ? 0 : (long)state_stride, as = nbatch == 1 ? 0 : (long)adj_stride;
+    const long slot = (long)nbatch * (nbatch == 1 ? S : (long)state_stride);  // one saved batch state
+    hipStream_t st = as_stream(stream);
+    auto state_at = [&](float *base) {
+        sw_ssg2d e = *g;
+        e.vx = base; e.vz = base + P; e.sxx = base + 2 * P; e.szz = base + 3 * P; e.sxz = base + 4 * P;
+        e.mem = base + 5 * P;
+        return e;
+    };
+    for (int t = 0; t < n; ++t) {
+        if (src->kind != SW_SRC_NONE)
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
+        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t) * slot, g->vx, slot * sizeof(float), hipMemcpyDeviceToDevice,
+                                  st));
+        if ((rc = launch_ssg2d_velocity_batch(g, nbatch, bs, st))) return rc;
+        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t + 1) * slot, g->vx, slot * sizeof(float),
+                                  hipMemcpyDeviceToDevice, st));
+        if ((rc = launch_ssg2d_stress_batch(g, nbatch, bs, st))) return rc;
+    }
+    for (int t = n - 1; t >= 0; --t) {
+        if (residual->nrec && (rc = launch_adj2d_inject_batch(adj, residual, it0 + t, nbatch, as, data_stride, st)))
+            return rc;
+        const sw_ssg2d mid = state_at(work + (2 * (size_t)t + 1) * slot);
+        if ((rc = launch_adj2d_stress_batch(&mid, adj, nbatch, bs, as, st))) return rc;
+        const sw_ssg2d pre = state_at(work + (2 * (size_t)t) * slot);
+        if ((rc = launch_adj2d_velocity_batch(&pre, adj, nbatch, bs, as, st))) return rc;
+    }
+    return launch_status("ssg2d adjoint segment batch");
+}
+
+int sw_ssg2d_model_gradient_batch(const sw_ssg2d *g, const sw_adj2d *adj, int32_t nbatch, int64_t adj_stride,
+                                  const float *vp, const float *vs, float *gvp, float *gvs, float *grho, void *stream)
+{
+    int rc = check_adj2d(g, adj);
+    if (rc) return rc;
+    SW_REQUIRE(nbatch >= 1 && nbatch <= SW_MAX_BATCH, "batch of %d shots outside 1..%d", nbatch, SW_MAX_BATCH);
+    SW_REQUIRE(vp && vs && gvp && gvs && grho, "null model or gradient arrays");
+    return launch_adj2d_model_gradient_batch(g, adj, nbatch, nbatch == 1 ? 0 : (long)adj_stride, vp, vs, gvp, gvs,
+                                             grho, as_stream(stream));
 }
 
 static int check_band(const sw_ssg3d *g, const sw_band *b)

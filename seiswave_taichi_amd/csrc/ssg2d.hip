@@ -50,8 +50,9 @@ struct Rot {
 };
 
 template <int N, int SCHEME>
-__global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g)
+__global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs)
 {
+    shift_state(g, (long)blockIdx.z * bs);
     const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
@@ -133,8 +134,9 @@ __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g)
 }
 
 template <int N, int SCHEME>
-__global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g)
+__global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs)
 {
+    shift_state(g, (long)blockIdx.z * bs);
     const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
@@ -222,35 +224,45 @@ __global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g)
     }
 }
 
-inline dim3 box_grid2(const sw_ssg2d &g, int N, dim3 blk)
+inline dim3 box_grid2(const sw_ssg2d &g, int N, dim3 blk, int nbatch)
 {
-    return dim3((g.nx - 2 * N + blk.x - 1) / blk.x, (g.nz - 2 * N + blk.y - 1) / blk.y, 1);
+    return dim3((g.nx - 2 * N + blk.x - 1) / blk.x, (g.nz - 2 * N + blk.y - 1) / blk.y, nbatch);
 }
 
 }  // namespace
 
-int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st)
+int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(g->order, {
-        const dim3 grd = box_grid2(*g, N, blk);
-        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS>), grd, blk, 0, st, *g);
-        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_SV>), grd, blk, 0, st, *g);
-        else hipLaunchKernelGGL((k_velocity2d<N, SW_2D_RSG>), grd, blk, 0, st, *g);
+        const dim3 grd = box_grid2(*g, N, blk, nbatch);
+        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS>), grd, blk, 0, st, *g, bs);
+        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_SV>), grd, blk, 0, st, *g, bs);
+        else hipLaunchKernelGGL((k_velocity2d<N, SW_2D_RSG>), grd, blk, 0, st, *g, bs);
     });
     return launch_status("ssg2d velocity");
 }
 
-int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st)
+int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(g->order, {
-        const dim3 grd = box_grid2(*g, N, blk);
-        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS>), grd, blk, 0, st, *g);
-        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_stress2d<N, SW_2D_SV>), grd, blk, 0, st, *g);
-        else hipLaunchKernelGGL((k_stress2d<N, SW_2D_RSG>), grd, blk, 0, st, *g);
+        const dim3 grd = box_grid2(*g, N, blk, nbatch);
+        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS>), grd, blk, 0, st, *g, bs);
+        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_stress2d<N, SW_2D_SV>), grd, blk, 0, st, *g, bs);
+        else hipLaunchKernelGGL((k_stress2d<N, SW_2D_RSG>), grd, blk, 0, st, *g, bs);
     });
     return launch_status("ssg2d stress");
+}
+
+int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st)
+{
+    return launch_ssg2d_velocity_batch(g, 1, 0, st);
+}
+
+int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st)
+{
+    return launch_ssg2d_stress_batch(g, 1, 0, st);
 }
 
 }  // namespace sw

@@ -90,6 +90,20 @@ __device__ __forceinline__ float cpml_apply(float D, float *m, float a, float b,
     return D / k + mm;
 }
 
+// Shot batches of the 2-D path: shot b of a batch keeps its state (and its
+// adjoint) at the descriptor's pointers plus b * stride floats; the model,
+// PML coefficients and flags are shared.
+__device__ __forceinline__ void shift_state(sw_ssg2d &g, long off)
+{
+    g.vx += off; g.vz += off; g.sxx += off; g.szz += off; g.sxz += off; g.mem += off;
+}
+
+__device__ __forceinline__ void shift_adj(sw_adj2d &q, long off)
+{
+    q.vx += off; q.vz += off; q.sxx += off; q.szz += off; q.sxz += off; q.mem += off; q.tmp += off;
+    q.g_b1 += off; q.g_b4 += off; q.g_lamh += off; q.g_muh += off; q.g_muz += off;
+}
+
 }  // namespace sw
 
 // Dispatch on the stencil half-length N (accuracy) with a compile-time constant.
