@@ -4,7 +4,7 @@ A "step" is one full reference time step (moment-tensor injection, velocity
 sweep, stress sweep, receiver gather) of the Elastic3D_SSG_Tensor class on a
 synthetic model that is resident in HBM before the timed region starts.
 
-Default workload (cfg2): 200^3 cells, 10 m spacing, homogeneous vp=3500,
+Default workload (tensor_200): 200^3 cells, 10 m spacing, homogeneous vp=3500,
 vs=2000, rho=2200, dt=1 ms, 20-cell CPML on six faces, seeded unit-norm moment
 tensor (10 Hz Ricker) at the centre, 50x50 receivers at k = s+5; 1500 steps.
 
@@ -43,7 +43,7 @@ def unit_mt(seed):
 
 def workload(name, nt, seed=0):
     """Model, geometry and PML of a named workload (host arrays)."""
-    if name in ("cfg2", "cfg2_spml"):
+    if name in ("tensor_200", "spml_200"):
         n = 200
         shape = (n, n, n)
         vs = np.full(shape, 2000.0, np.float32)
@@ -54,7 +54,7 @@ def workload(name, nt, seed=0):
         lin = np.round(np.linspace(3, n - 4, 50)).astype(np.int64)
         rx, ry = (a.ravel() for a in np.meshgrid(lin, lin, indexing="ij"))
         pml = pml_params(3500.0, 2000.0, npml, d, f, kmax=4.0)
-    elif name == "cfg3":
+    elif name == "layered_600x600x400":
         from seiswave_taichi_amd.models import layered_folded_model
         shape = (600, 600, 400)
         d, f, npml = (12.5, 12.5, 12.5), 10.0, 20
@@ -198,7 +198,7 @@ def gradient_workload(args):
     cells = nx * nz * nt * len(shots)
     out = {"metric": "2D adjoint gradient seconds", "value": round(t_all, 3), "unit": "s/gradient",
            "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "cfg5_gradient: 800x300 2-D VS, 16 explosive shots, 400 receivers (vx, vz), "
+           "config": {"workload": "gradient_800x300: 800x300 2-D VS, 16 explosive shots, 400 receivers (vx, vz), "
                                   "2500 steps, checkpoint 50, L2 misfit", "dt": dt},
            "forward_s": round(t_fwd, 3), "backward_s": round(t_all - t_fwd, 3),
            "forward_Gcells_per_s": round(cells / t_fwd / 1e9, 3),
@@ -213,9 +213,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="cfg2", choices=["cfg2", "cfg2_spml", "cfg3", "cfg5_gradient"])
+    ap.add_argument("--config", default="tensor_200",
+                    choices=["tensor_200", "spml_200", "layered_600x600x400", "gradient_800x300"])
     ap.add_argument("--fd-cells", type=int, default=4,
-                    help="cfg5_gradient: cells per parameter of the finite-difference spot check")
+                    help="gradient_800x300: cells per parameter of the finite-difference spot check")
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -227,7 +228,7 @@ def main():
                                    "a design, not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    if args.config == "cfg5_gradient":
+    if args.config == "gradient_800x300":
         return gradient_workload(args)
 
     import torch
@@ -244,7 +245,7 @@ def main():
     rx, ry, rz = w["rec"]
     amps = ricker2(np.arange(nt), w["dt"], 1.5 / w["f"], w["f"], 1.0)
     src = {"kind": "moment", "cell": w["src"], "mt": w["mt"], "wavelet": amps}
-    spml = args.config == "cfg2_spml"
+    spml = args.config == "spml_200"
     if spml:
         sim = SPML(w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], *w["src"], 0, 0, 0, nt, 2, w["f"],
                    source=src, receivers=np.stack([rx, ry, rz], 1))
