@@ -1,0 +1,255 @@
+// Device helpers of the 3-D sweep kernels (ssg3d.hip): addressing, stencil
+// taps, staggered differences, CPML slabs and the velocity-sweep register
+// plane.  They sit in an anonymous namespace, so a translation unit that
+// includes them keeps its own copy.
+#pragma once
+
+#include "sw_internal.h"
+
+namespace sw {
+
+namespace {
+
+constexpr int kXCD = 8;  // XCDs per MI355X; any value keeps the renumbering a bijection
+
+__device__ __forceinline__ int xcd_logical_block(int b, int nb)
+{
+    const int q = nb / kXCD, r = nb % kXCD;
+    const int x = b % kXCD, s = b / kXCD;
+    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+
+// Fields are addressed by 32-bit byte offsets from their base pointers: one
+// offset register per column serves every field, and the wave-uniform tap
+// deltas fold into the load instructions.  The host keeps every array below
+// 4 GiB.
+__device__ __forceinline__ float ld(const float *__restrict__ base, uint32_t off)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+
+__device__ __forceinline__ void st(float *__restrict__ base, uint32_t off, float v)
+{
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// Load at byte offset v + d, d a wave-uniform non-negative delta.
+__device__ __forceinline__ float ldu(const float *__restrict__ base, uint32_t v, int d)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + v + (uint32_t)d);
+}
+
+// Base pointers of all the arrays a 3-D sweep touches.
+struct Res {
+    float *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz, *mx, *my, *mz;
+    const float *rho, *lam, *mu;
+};
+
+__device__ __forceinline__ Res make_res(const sw_ssg3d &g)
+{
+    Res r;
+    r.vx = g.vx; r.vy = g.vy; r.vz = g.vz;
+    r.sxx = g.sxx; r.syy = g.syy; r.szz = g.szz;
+    r.sxy = g.sxy; r.sxz = g.sxz; r.syz = g.syz;
+    r.rho = g.rho; r.lam = g.lam; r.mu = g.mu;
+    r.mx = g.mem_x; r.my = g.mem_y; r.mz = g.mem_z;
+    return r;
+}
+
+// Tap arrays: a "bm" array holds f[p + (t-N)*st] (offsets -N..N-1), an "fm"
+// array f[p + (t-N+1)*st] (offsets -N+1..N), t = 0..2N-1.  All loads of a
+// plane are addressed from vb = byte offset of (i-N, j-N, k); c0 is the
+// uniform delta of (i, j, k) from there and st the tap stride, in bytes.
+template <int N>
+__device__ __forceinline__ void taps_bm(float *w, const float *__restrict__ f, uint32_t vb, int c0, int st)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t) w[t] = ldu(f, vb, c0 + (t - N) * st);
+}
+
+template <int N>
+__device__ __forceinline__ void taps_fm(float *w, const float *__restrict__ f, uint32_t vb, int c0, int st)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t) w[t] = ldu(f, vb, c0 + (t - N + 1) * st);
+}
+
+// x taps are 2N consecutive floats: one 16-byte (or 8-byte) load per 4 (or 2)
+// taps instead of one load each.  The addresses are only 4-byte aligned,
+// which global vector loads accept.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+struct __attribute__((packed, aligned(4))) F4U { f4v v; };
+struct __attribute__((packed, aligned(4))) F2U { f2v v; };
+
+template <int M>
+__device__ __forceinline__ void ld_run(float *w, const float *__restrict__ f, uint32_t v, int d)
+{
+    const char *b = reinterpret_cast<const char *>(f) + v + (uint32_t)d;
+    int t = 0;
+#pragma unroll
+    for (; t + 4 <= M; t += 4) {
+        const f4v x = reinterpret_cast<const F4U *>(b + 4 * t)->v;
+        w[t] = x.x; w[t + 1] = x.y; w[t + 2] = x.z; w[t + 3] = x.w;
+    }
+#pragma unroll
+    for (; t + 2 <= M; t += 2) {
+        const f2v x = reinterpret_cast<const F2U *>(b + 4 * t)->v;
+        w[t] = x.x; w[t + 1] = x.y;
+    }
+#pragma unroll
+    for (; t < M; ++t) w[t] = *reinterpret_cast<const float *>(b + 4 * t);
+}
+
+template <int N>
+__device__ __forceinline__ void xtaps_bm(float *w, const float *__restrict__ f, uint32_t vb, int c0)
+{
+    ld_run<2 * N>(w, f, vb, c0 - N * 4);
+}
+
+template <int N>
+__device__ __forceinline__ void xtaps_fm(float *w, const float *__restrict__ f, uint32_t vb, int c0)
+{
+    ld_run<2 * N>(w, f, vb, c0 + (1 - N) * 4);
+}
+
+// Staggered differences from tap arrays, accumulated exactly as
+// src/Differential3D.py:7-44 does (0, then c[id]*(difference) in id order).
+template <int N>
+__device__ __forceinline__ float diff_bm(const float *w, const float *c)
+{
+    float v = 0.f;
+#pragma unroll
+    for (int id = 0; id < N; ++id) v += c[id] * (w[N + id] - w[N - id - 1]);
+    return v;
+}
+
+template <int N>
+__device__ __forceinline__ float diff_fm(const float *w, const float *c)
+{
+    float v = 0.f;
+#pragma unroll
+    for (int id = 0; id < N; ++id) v += c[id] * (w[N + id] - w[N - 1 - id]);
+    return v;
+}
+
+template <int N>
+__device__ __forceinline__ void win_shift(float *w)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N - 1; ++t) w[t] = w[t + 1];
+}
+
+template <int M>
+__device__ __forceinline__ void copy_n(float *d, const float *s)
+{
+#pragma unroll
+    for (int t = 0; t < M; ++t) d[t] = s[t];
+}
+
+// Byte offsets of one column into the CPML slab arrays.
+struct Slab {
+    int xs, ys;
+    uint32_t rx, ry;      // x / y slab offset at k = 0
+    uint32_t sx, sy, sz;  // per-k strides of the x / y slab arrays; plane size of the z slab
+    uint32_t nxs, nys, nzs;
+};
+
+template <int PM>
+__device__ __forceinline__ Slab slab_setup(const sw_ssg3d &g, int i, int j)
+{
+    Slab s;
+    const int wx = g.px.lo + (g.nx - g.px.hi);
+    const int wy = g.py.lo + (g.ny - g.py.hi);
+    const int wz = g.pz.lo + (g.nz - g.pz.hi);
+    s.xs = (PM & 1) ? slab_index(i, g.px.lo, g.px.hi) : -1;
+    s.ys = (PM & 2) ? slab_index(j, g.py.lo, g.py.hi) : -1;
+    s.rx = 4u * (uint32_t)(j * wx + s.xs);
+    s.ry = 4u * (uint32_t)(s.ys * g.nx + i);
+    s.sx = 4u * (uint32_t)(g.ny * wx);
+    s.sy = 4u * (uint32_t)(wy * g.nx);
+    s.sz = 4u * (uint32_t)(g.ny * g.nx);
+    s.nxs = s.sx * (uint32_t)g.nz;
+    s.nys = s.sy * (uint32_t)g.nz;
+    s.nzs = s.sz * (uint32_t)wz;
+    return s;
+}
+
+// CPML coefficients of one point: integer (a, b, k) and half (ah, bh, kh).
+struct Coef {
+    float a, b, k, ah, bh, kh;
+};
+
+__device__ __forceinline__ Coef coef_of(const sw_axis &ax, int i, bool in_slab)
+{
+    Coef c{0.f, 1.f, 1.f, 0.f, 1.f, 1.f};
+    if (in_slab) {
+        c.a = ax.a[i]; c.b = ax.b[i]; c.k = ax.k[i];
+        c.ah = ax.a_h[i]; c.bh = ax.b_h[i]; c.kh = ax.k_h[i];
+    }
+    return c;
+}
+
+// m = b*m + a*D; D = D/k + m  (src/Elastic3D_SSG_CPML.py:172-177)
+__device__ __forceinline__ float cpml_step(float D, float &m, float a, float b, float k)
+{
+    m = b * m + a * D;
+    return D / k + m;
+}
+
+// ------------------------------------------------------------ velocity sweep
+template <int N>
+struct VPlane {
+    float sxx[2 * N];   // fm taps along x (Dx3fm, vx)
+    float sxy_x[2 * N]; // bm taps along x (Dx3bm, vy)
+    float sxy_y[2 * N]; // bm taps along y (Dy3bm, vx)
+    float syy[2 * N];   // fm taps along y (Dy3fm, vy)
+    float sxz_x[2 * N]; // bm taps along x (Dx3bm, vz)
+    float syz_y[2 * N]; // bm taps along y (Dy3bm, vz)
+    float rho[4];       // p, p+SY, p+1+SY, p+1
+    float v[3];
+    float wxz, wyz, wzz; // next window entries: sxz, syz at k+N-1, szz at k+N
+    float mx[3], my[3], mz[3];
+    Coef cz;
+    int zs;
+};
+
+// p: byte offset of (i, j, k); pc: of (i, j, 0); SY, SZ: byte strides.
+template <int N, int PM>
+__device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
+                                            const Slab &sl)
+{
+    // vb: byte offset of (i-N, j-N, k); every load below is vb + a uniform delta
+    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
+    const int c0 = N * 4 + N * SY;
+    xtaps_fm<N>(P.sxx, R.sxx, vb, c0);
+    xtaps_bm<N>(P.sxy_x, R.sxy, vb, c0);
+    taps_bm<N>(P.sxy_y, R.sxy, vb, c0, SY);
+    taps_fm<N>(P.syy, R.syy, vb, c0, SY);
+    xtaps_bm<N>(P.sxz_x, R.sxz, vb, c0);
+    taps_bm<N>(P.syz_y, R.syz, vb, c0, SY);
+    P.rho[0] = ldu(R.rho, vb, c0);
+    P.rho[1] = ldu(R.rho, vb, c0 + SY);
+    P.rho[2] = ldu(R.rho, vb, c0 + 4 + SY);
+    P.rho[3] = ldu(R.rho, vb, c0 + 4);
+    P.v[0] = ldu(R.vx, vb, c0);
+    P.v[1] = ldu(R.vy, vb, c0);
+    P.v[2] = ldu(R.vz, vb, c0);
+    P.wxz = ldu(R.sxz, vb, c0 + (N - 1) * (int)SZ);
+    P.wyz = ldu(R.syz, vb, c0 + (N - 1) * (int)SZ);
+    P.wzz = ldu(R.szz, vb, c0 + N * (int)SZ);
+    P.zs = (PM & 4) ? slab_index(k, g.pz.lo, g.pz.hi) : -1;
+    P.cz = coef_of(g.pz, k, P.zs >= 0);
+    const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+    const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        P.mx[m] = sl.xs >= 0 ? ldu(R.mx, ox, (int)(m * sl.nxs)) : 0.f;
+        P.my[m] = sl.ys >= 0 ? ldu(R.my, oy, (int)(m * sl.nys)) : 0.f;
+        P.mz[m] = P.zs >= 0 ? ldu(R.mz, oz, (int)(m * sl.nzs)) : 0.f;
+    }
+}
+
+}  // namespace
+
+}  // namespace sw
