@@ -143,6 +143,13 @@ int sw_ssg3d_record(const sw_ssg3d *g, const sw_receivers *rec, int32_t it, void
  * rec may be NULL. */
 int sw_ssg3d_run(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec,
                  int32_t it0, int32_t nsteps, void *stream);
+/* sw_ssg3d_run with options (no counterpart in the reference's step methods):
+ *   SW_RUN_FREE_SURFACE  sw_ssg3d_free_surface before every step, as the
+ *                        reference's 2-D drivers call AEA() before each step
+ *                        (Relay_wave.py:55-58). */
+#define SW_RUN_FREE_SURFACE 1
+int sw_ssg3d_run_ex(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec,
+                    int32_t it0, int32_t nsteps, int32_t flags, void *stream);
 
 /* ---- 2-D (src/Elastic2D_CPML.py) ---- */
 #define SW_2D_VS 0  /* update_SSG_VS :127-200 */
@@ -175,6 +182,9 @@ int sw_ssg2d_free_surface_init(const sw_ssg2d *g, void *stream);
 int sw_ssg2d_free_surface(const sw_ssg2d *g, void *stream);
 /* 3-D variant (src/Elastic3D_SSG_Tensor.py:151-157): lam = 0, rho *= 0.5 on k < order. */
 int sw_ssg3d_free_surface_init(const sw_ssg3d *g, float *rho, float *lam, void *stream);
+/* 3-D free surface per step (src/Elastic3D_SSG_Tensor.py:159-163, a ti.func the
+ * reference never calls): szz = 0 on plane k = order. */
+int sw_ssg3d_free_surface(const sw_ssg3d *g, void *stream);
 
 /* ---- 2-D discrete adjoint (no counterpart in the reference; the gradient
  * path of a seismogram misfit with respect to vp, vs, rho) ----

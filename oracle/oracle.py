@@ -243,6 +243,11 @@ class Oracle3D:
         """Elastic3D_SSG_Tensor.AEA_init (src/Elastic3D_SSG_Tensor.py:151-157)."""
         self.lib.or3d_aea_init(C.byref(self.st), _ptr(self.lib, self.rho), _ptr(self.lib, self.lam))
 
+    def aea(self):
+        """Elastic3D_SSG_Tensor.AEA (src/Elastic3D_SSG_Tensor.py:159-163):
+        szz[i, j, accuracy] = 0 for every i, j."""
+        self.fields["szz"][:, :, self.s] = 0
+
     def wavelet(self, it: int) -> float:
         """The reference's per-step source amplitude for this class."""
         if self.kind == "cpml":    # src/Elastic3D_SSG_CPML.py:245

@@ -3,7 +3,7 @@
 Same constructor (src/Elastic3D_SSG_Tensor.py:9-28) with receiver index fields
 ``rsx, rsy, rsz`` of shape (1, N) and a (3, 3) moment tensor ``MT``;
 ``SetADEPML3D`` (:361-559), ``update_SSG(nt)`` (:166-345), ``AEA_init``
-(:151-157).  Step order: moment-tensor injection
+(:151-157) and ``AEA`` (:159-163, working here).  Step order: moment-tensor injection
 ``s_ab[isx,isy,isz] += Ricker2(nt, dt, 0.03, f0, src_scale)*(-M_ab)`` for the
 upper triangle of MT (:186-193), velocity sweep, stress sweep, then
 ``data[nt, r] = vx[rsx[0,r], rsy[0,r], rsz[0,r]]`` (:344-345).
@@ -67,6 +67,10 @@ class ElasticWAVE(ElasticWAVE3DBase):
             src[:, :, :e.star] *= 0.5
 
     def AEA(self):
-        """src/Elastic3D_SSG_Tensor.py:159-163 is a ti.func that update_SSG never
-        calls; kept as a no-op for interface parity."""
-        return None
+        """src/Elastic3D_SSG_Tensor.py:159-163: szz[:, :, accuracy] = 0.  The
+        reference's is a ti.func that update_SSG never calls (its 3-D free
+        surface is dead code); here it is a working per-step call, to be made
+        before each update_SSG as the 2-D drivers do (Relay_wave.py:55-58), or
+        ``run(..., free_surface=True)`` does it inside the native step loop."""
+        e = self._e
+        L.check(e.lib.sw_ssg3d_free_surface(e.descriptor(), e.stream()), "sw_ssg3d_free_surface")

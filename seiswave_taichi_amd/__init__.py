@@ -8,6 +8,7 @@ package, with the time step in hand-written HIP kernels for gfx950
     from seiswave_taichi_amd.Elastic3D_SSG_Tensor import ElasticWAVE
     from seiswave_taichi_amd.Elastic3D_SSG_SPML import ElasticWAVE
     from seiswave_taichi_amd.Elastic2D_CPML import ElasticWAVE
+    from seiswave_taichi_amd.Elastic2D_sparse import ElasticWAVE
 
 ``install_src_alias()`` makes ``from src.<Module> import ElasticWAVE`` resolve
 to these modules, so reference driver scripts run unchanged.
@@ -17,7 +18,8 @@ import sys
 
 __version__ = "0.1.0"
 
-_MODULES = ("Elastic2D_CPML", "Elastic3D_SSG_CPML", "Elastic3D_SSG_SPML", "Elastic3D_SSG_Tensor")
+_MODULES = ("Elastic2D_CPML", "Elastic2D_sparse", "Elastic3D_SSG_CPML", "Elastic3D_SSG_SPML",
+            "Elastic3D_SSG_Tensor")
 
 
 def install_src_alias() -> None:

@@ -87,16 +87,21 @@ class ElasticWAVE3DBase:
                 setattr(self, ref.format(name), arr)
 
     # -------------------------------------------------------------- steps
-    def _run(self, it0: int, nsteps: int):
+    def _run(self, it0: int, nsteps: int, flags: int = 0):
         e = self._e
         g = e.descriptor()
         if it0 < 0 or it0 + nsteps > self.nt:
             raise IndexError(f"steps {it0}..{it0 + nsteps - 1} outside 0..{self.nt - 1}")
-        L.check(e.lib.sw_ssg3d_run(g, self._src.s, self._rec.r, it0, nsteps, e.stream()), "sw_ssg3d_run")
+        L.check(e.lib.sw_ssg3d_run_ex(g, self._src.s, self._rec.r, it0, nsteps, flags, e.stream()),
+                "sw_ssg3d_run_ex")
 
-    def run(self, nsteps: int | None = None, it0: int = 0):
-        """Extension: ``nsteps`` reference steps from ``it0`` in one native call."""
-        self._run(it0, self.nt - it0 if nsteps is None else nsteps)
+    def run(self, nsteps: int | None = None, it0: int = 0, free_surface: bool = False):
+        """Extension: ``nsteps`` reference steps from ``it0`` in one native call.
+        ``free_surface``: szz = 0 on plane k = accuracy before every step (the
+        3-D counterpart of the reference's 2-D AEA() before each step;
+        AEA_init() sets the material side)."""
+        self._run(it0, self.nt - it0 if nsteps is None else nsteps,
+                  L.SW_RUN_FREE_SURFACE if free_surface else 0)
 
     def reset(self):
         """Extension: zero the wavefields, boundary state and record."""

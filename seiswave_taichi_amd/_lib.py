@@ -16,6 +16,7 @@ SW_MAX_ORDER = 8
 SW_CLASS_CPML, SW_CLASS_TENSOR, SW_CLASS_SPML = 0, 1, 2
 SW_SRC_NONE, SW_SRC_FORCE_VZ, SW_SRC_MOMENT, SW_SRC_EXPLOSIVE, SW_SRC_FORCE_VX = 0, 1, 2, 3, 4
 SW_2D_VS, SW_2D_SV, SW_2D_RSG = 0, 1, 2
+SW_RUN_FREE_SURFACE = 1
 FIELD_CODE = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7, "syz": 8}
 
 _fp = C.POINTER(C.c_float)
@@ -83,6 +84,7 @@ SIGNATURES = {
     "sw_ssg3d_inject": [_P3, _PS, C.c_int32, C.c_void_p],
     "sw_ssg3d_record": [_P3, _PR, C.c_int32, C.c_void_p],
     "sw_ssg3d_run": [_P3, _PS, _PR, C.c_int32, C.c_int32, C.c_void_p],
+    "sw_ssg3d_run_ex": [_P3, _PS, _PR, C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
     "sw_ssg2d_velocity": [_P2, C.c_void_p],
     "sw_ssg2d_stress": [_P2, C.c_void_p],
     "sw_ssg2d_inject": [_P2, _PS, C.c_int32, C.c_void_p],
@@ -91,6 +93,7 @@ SIGNATURES = {
     "sw_ssg2d_free_surface_init": [_P2, C.c_void_p],
     "sw_ssg2d_free_surface": [_P2, C.c_void_p],
     "sw_ssg3d_free_surface_init": [_P3, _fp, _fp, C.c_void_p],
+    "sw_ssg3d_free_surface": [_P3, C.c_void_p],
     "sw_ssg2d_adjoint_stress": [_P2, _PA, C.c_void_p],
     "sw_ssg2d_adjoint_velocity": [_P2, _PA, C.c_void_p],
     "sw_ssg2d_adjoint_inject": [_P2, _PA, _PR, C.c_int32, C.c_void_p],

@@ -285,6 +285,21 @@ __global__ void k_free_surface2d(int nx, int nz, int order, float *sxx)
     if (j < nz) sxx[(long)j * nx + order] = 0.f;
 }
 
+// src/Elastic3D_SSG_Tensor.py:159-163: szz[i, j, order] = 0 (plane k = order)
+__global__ void k_free_surface3d(int nx, int ny, int order, float *szz)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long)nx * ny) szz[(long)order * nx * ny + t] = 0.f;
+}
+
+int free_surface3d_launch(const sw_ssg3d *g, hipStream_t st)
+{
+    const long n = (long)g->nx * g->ny;
+    hipLaunchKernelGGL(k_free_surface3d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g->nx, g->ny, g->order,
+                       g->szz);
+    return launch_status("3-D free surface");
+}
+
 }  // namespace
 }  // namespace sw
 
@@ -359,6 +374,12 @@ int sw_ssg3d_record(const sw_ssg3d *g, const sw_receivers *rec, int32_t it, void
 int sw_ssg3d_run(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, int32_t it0, int32_t nsteps,
                  void *stream)
 {
+    return sw_ssg3d_run_ex(g, src, rec, it0, nsteps, 0, stream);
+}
+
+int sw_ssg3d_run_ex(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, int32_t it0,
+                    int32_t nsteps, int32_t flags, void *stream)
+{
     const bool spml = g && g->cls == SW_CLASS_SPML;
     int rc = check_ssg3d(g, spml);
     if (rc) return rc;
@@ -366,10 +387,12 @@ int sw_ssg3d_run(const sw_ssg3d *g, const sw_source *src, const sw_receivers *re
     if (nsteps == 0) return SW_OK;
     if ((rc = check_source3d(g, src, it0)) || (rc = check_source3d(g, src, it0 + nsteps - 1))) return rc;
     if (rec && ((rc = check_receivers(rec, it0)) || (rc = check_receivers(rec, it0 + nsteps - 1)))) return rc;
+    SW_REQUIRE((flags & ~SW_RUN_FREE_SURFACE) == 0, "unknown run flags 0x%x", flags);
     hipStream_t st = as_stream(stream);
     const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
     const FieldPtrs fp = fields3d(g);
     for (int it = it0; it < it0 + nsteps; ++it) {
+        if ((flags & SW_RUN_FREE_SURFACE) && (rc = free_surface3d_launch(g, st))) return rc;
         if (pre && src->kind != SW_SRC_NONE) {
             hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
         }
@@ -459,6 +482,13 @@ int sw_ssg2d_free_surface(const sw_ssg2d *g, void *stream)
     hipLaunchKernelGGL(k_free_surface2d, dim3((g->nz + 255) / 256), dim3(256), 0, as_stream(stream), g->nx, g->nz,
                        g->order, g->sxx);
     return launch_status("2-D free surface");
+}
+
+int sw_ssg3d_free_surface(const sw_ssg3d *g, void *stream)
+{
+    SW_REQUIRE(g && g->szz, "null argument");
+    SW_REQUIRE(g->order >= 1 && g->order < g->nz, "bad order");
+    return free_surface3d_launch(g, as_stream(stream));
 }
 
 int sw_ssg3d_free_surface_init(const sw_ssg3d *g, float *rho, float *lam, void *stream)

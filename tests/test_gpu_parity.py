@@ -282,3 +282,76 @@ def test_config2_shape_short_run_matches_oracle():
     for name in ("vx", "vz", "sxx", "syz"):
         np.testing.assert_array_equal(getattr(w, name).to_numpy(), o.fields[name], err_msg=name)
     torch.cuda.synchronize()
+
+
+def test_3d_free_surface_matches_oracle():
+    """Tensor class with AEA_init and a working AEA (szz = 0 on plane
+    k = accuracy before every step), per step and inside the native run loop,
+    against the oracle doing the same."""
+    from seiswave_taichi_amd.Elastic3D_SSG_Tensor import ElasticWAVE
+    shape, order, steps, f, dt = (30, 28, 32), 2, 40, 20.0, 5e-4
+    d = (10.0, 10.0, 10.0)
+    vs, vp, rho = layered_model(shape, seed=12)
+    src = (15, 14, 6)
+    rx = np.arange(4, 26, 3)
+    ry, rz = np.full_like(rx, 14), np.full_like(rx, order + 1)
+    m = unit_mt(3)
+    mt3 = np.zeros((3, 3), np.float32)
+    mt3[0, 0], mt3[1, 1], mt3[2, 2], mt3[0, 1], mt3[0, 2], mt3[1, 2] = m
+    p = _pml(vp, vs, 5, d, f)
+    surface = [True, True, True, True, False, True]
+    runs = []
+    for native in (False, True):
+        rho_w = rho.copy()
+        w = ElasticWAVE(vs, vp, rho_w, *d, dt, *src, rx[None], ry[None], rz[None], mt3, 1.0, steps, order, f)
+        w.SetADEPML3D(surface, p)
+        w.AEA_init()
+        np.testing.assert_array_equal(rho_w[:, :, :order], rho[:, :, :order] * np.float32(0.5))
+        if native:
+            w.run(steps, 0, free_surface=True)
+        else:
+            for it in range(steps):
+                w.AEA()
+                w.update_SSG(it)
+        runs.append(w)
+    o = O.Oracle3D("tensor", vs, vp, rho, *d, dt, src, np.stack([rx, ry, rz], 1), steps, order, freq=f, mt=m)
+    o.set_pml(surface, p)
+    o.aea_init()
+    amps = ricker2(np.arange(steps), dt, 0.03, f, 1.0)
+    for it in range(steps):
+        o.aea()
+        o.step(it, float(amps[it]))
+    for w in runs:
+        for name in o.FIELDS:
+            np.testing.assert_array_equal(getattr(w, name).to_numpy(), o.fields[name], err_msg=name)
+        np.testing.assert_array_equal(w.data.to_numpy().reshape(o.data.shape), o.data)
+    assert np.abs(o.fields["vz"]).max() > 0
+
+
+def test_2d_sparse_class_matches_oracle():
+    """Elastic2D_sparse: the VS step with lam = rho*(vp^2 - vs^2) and the
+    Ricker(f0, 1) force of src/Elastic2D_sparse.py:105-113,171; SV and RSG
+    raise AttributeError as the reference's do."""
+    from seiswave_taichi_amd.Elastic2D_sparse import ElasticWAVE
+    shape, order, steps, f, dt = (48, 44), 3, 90, 25.0, 6e-4
+    d = (10.0, 10.0)
+    vs, vp, rho = layered_model(shape, seed=13)
+    w = ElasticWAVE(vs, vp, rho, *d, dt, 20, 22, 30, 9, steps, order, f)
+    p = pml_dict(float(vp.max()), 6, d, f, float(vs.max()), dims=2)
+    w.SetADEPML2D([True, True, True, True], p)
+    for it in range(steps):
+        w.update_SSG_VS(it)
+    o = O.Oracle2D(vs, vp, rho, *d, dt, 20, 22, 30, 9, steps, order, f)
+    np.copyto(o.lam, (rho * (vp ** 2 - vs ** 2)).astype(np.float32))
+    o.set_pml([True, True, True, True], p)
+    amps = ricker(np.arange(steps), dt, f, 1.0)
+    for it in range(steps):
+        o.vs_step(it, float(amps[it]))
+    for name in o.FIELDS:
+        np.testing.assert_array_equal(getattr(w, name).to_numpy(), o.fields[name], err_msg=name)
+    np.testing.assert_array_equal(w.data.to_numpy(), o.data)
+    np.testing.assert_array_equal(w.lam.to_numpy(), rho * (vp ** 2 - vs ** 2))
+    with pytest.raises(AttributeError):
+        w.update_SSG_SV(0)
+    with pytest.raises(AttributeError):
+        w.update_RSG(0, 1.0)
