@@ -200,6 +200,13 @@ typedef struct sw_adj2d {
     float *g_b1, *g_b4;               /* d misfit / d(dt/rho) at vx, vz cells      */
     float *g_lamh, *g_muh, *g_muz;    /* sxx cells: d/d lam_h and d/d mu_h - 2 d/d lam_h;
                                          sxz cells: d/d mu_z                        */
+    float *g_src;                     /* force sources only: g_src[b] += amp[it] * the
+                                         adjoint of the forced velocity of shot b at
+                                         its source cell, summed over the steps (the
+                                         force dt*amp/rho[p] then adds
+                                         -dt*g_src[b]/rho[p]^2 to d misfit/d rho[p]);
+                                         indexed by the shot, not shifted by
+                                         adj_stride; may be NULL for stress sources */
 } sw_adj2d;
 
 int sw_ssg2d_adjoint_stress(const sw_ssg2d *fwd, const sw_adj2d *adj, void *stream);
@@ -221,20 +228,21 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
  * the same calls as sw_ssg2d_adjoint_inject(residual, it),
  * sw_ssg2d_adjoint_stress (state after the velocity sweep) and
  * sw_ssg2d_adjoint_velocity (state before it).  src: explosive or moment
- * tensor (or none). */
+ * tensor (injected before the velocity sweep), a force SW_SRC_FORCE_VZ/VX
+ * (after it, as sw_ssg2d_run does; needs adj->g_src), or none. */
 int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_adj2d *adj,
                              const sw_receivers *residual, float *work, int32_t it0, int32_t n, void *stream);
 
 /* ---- shot batches (no counterpart in the reference): nbatch shots of the
  * SW_2D_VS scheme in each launch.  Shot b's state is the 15-plane block at
  * g->vx + b*state_stride (layout as for sw_ssg2d_adjoint_segment), its
- * source is src (explosive or moment tensor, shared wavelet) at cell
+ * source is src (any kind of sw_ssg2d_adjoint_segment, shared wavelet) at cell
  * (cells[2b], cells[2b+1]), its traces / residuals are rec->data +
  * b*data_stride, and its adjoint block is every sw_adj2d pointer +
  * b*adj_stride (24 planes: 15 state, 4 scratch, 5 accumulators).  The
  * model, PML and receiver positions are shared. */
 #define SW_MAX_BATCH 64
-/* nsteps of sw_ssg2d_run (stress source before the velocity sweep) for every shot */
+/* nsteps of sw_ssg2d_run for every shot */
 int sw_ssg2d_run_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
                        const int32_t *cells, const sw_receivers *rec, int64_t data_stride, int32_t it0,
                        int32_t nsteps, void *stream);

@@ -51,7 +51,7 @@ class SwSSG2D(C.Structure):
 
 class SwAdj2D(C.Structure):
     _fields_ = [(n, _fp) for n in ("vx", "vz", "sxx", "szz", "sxz", "mem", "tmp", "g_b1", "g_b4",
-                                   "g_lamh", "g_muh", "g_muz")]
+                                   "g_lamh", "g_muh", "g_muz", "g_src")]
 
 
 class SwSource(C.Structure):

@@ -13,9 +13,11 @@ checkpoint, keeping the state before and after the velocity stage of every
 step of the segment, then runs the adjoint backwards through it, one native
 call per segment (sw_ssg2d_adjoint_segment).
 
-Sources inject into the stresses (explosive or moment tensor), which do not
-depend on the model; force sources are refused because their 1/rho factor is
-not part of the adjoint.
+Sources: explosive or moment tensor (added to the stresses before the
+velocity sweep; no model dependence), or a force on vz or vx (added after the
+velocity sweep as dt*amp/rho[p], src/Elastic2D_CPML.py:170,245): the adjoint
+then also accumulates, per shot, amp times the adjoint of the forced velocity
+at the source cell, which contributes -dt*sum/rho[p]**2 to d/d rho at p.
 """
 from __future__ import annotations
 
@@ -81,9 +83,10 @@ class Elastic2DModeling:
             if cn not in _FIELD_ROW:
                 raise ValueError(f"unknown component {cn}")
         if isinstance(source, str):
-            if source != "explosive":
-                raise NotImplementedError("the adjoint covers stress sources (explosive or moment tensor)")
-            self.src_kind, mt = L.SW_SRC_EXPLOSIVE, np.zeros(6, np.float32)
+            kinds = {"explosive": L.SW_SRC_EXPLOSIVE, "force_vz": L.SW_SRC_FORCE_VZ, "force_vx": L.SW_SRC_FORCE_VX}
+            if source not in kinds:
+                raise ValueError(f"source {source!r}: explosive, force_vz, force_vx or (Mxx, Mzz, Mxz)")
+            self.src_kind, mt = kinds[source], np.zeros(6, np.float32)
         else:
             mxx, mzz, mxz = (float(v) for v in source)
             self.src_kind, mt = L.SW_SRC_MOMENT, np.array([mxx, 0, mzz, 0, mxz, 0], np.float32)
@@ -92,6 +95,7 @@ class Elastic2DModeling:
         self.amp = torch.from_numpy(np.ascontiguousarray(w, dtype=np.float32)).to(dev)
         # model-dependent device arrays are set by set_model
         self.rho = self.lam = self.mu = None
+        self.force = self.src_kind in (L.SW_SRC_FORCE_VZ, L.SW_SRC_FORCE_VX)
 
     # -------------------------------------------------------------- plumbing
     def set_model(self, vp: torch.Tensor, vs: torch.Tensor, rho: torch.Tensor):
@@ -221,11 +225,13 @@ class Elastic2DModeling:
         ncomp = len(self.components)
         out = torch.empty((3, nz, nx), dtype=torch.float32, device=self.device)
         total = None
+        g_src = torch.zeros(len(self.shots), dtype=torch.float32, device=self.device)
         for group, ck in zip(self._groups(), checkpoints):
             nb = len(group)
             # per shot: 15 adjoint state planes, 4 scratch, 5 accumulators
             adjb = torch.zeros((nb, 24, nz, nx), dtype=torch.float32, device=self.device)
-            q = self._adj_desc(adjb[0, :NSTATE], adjb[0, NSTATE:NSTATE + 4], adjb[0, NSTATE + 4:])
+            q = self._adj_desc(adjb[0, :NSTATE], adjb[0, NSTATE:NSTATE + 4], adjb[0, NSTATE + 4:],
+                               g_src[group[0]:] if self.force else None)
             work = self.new_state(nb)
             seg = self.new_state(2 * K * nb)
             g = self.desc(work[0])
@@ -245,7 +251,16 @@ class Elastic2DModeling:
                     "sw_ssg2d_model_gradient_batch")
             total = part if total is None else total + part
         out.copy_(total)
+        self._force_gradient(out[2], g_src)
         return tuple(out[i].t() for i in range(3))
+
+    def _force_gradient(self, grho, g_src):
+        """d/d rho[p] of the forces dt*amp/rho[p]: -dt*g_src/rho[p]**2 per shot
+        (grho in device layout [nz][nx])."""
+        if not self.force:
+            return
+        for s, (ix, iz) in enumerate(self.shots):
+            grho[iz, ix] += -float(self.dt) * g_src[s] / (self.rho[iz, ix] * self.rho[iz, ix])
 
     # -------------------------------------------------------------- backward
     def gradient(self, grad_data: torch.Tensor, checkpoints):
@@ -260,9 +275,10 @@ class Elastic2DModeling:
         tmp = torch.zeros((4, nz, nx), dtype=torch.float32, device=self.device)
         seg = self.new_state(2 * K)  # states before / after the velocity sweep of each step of a segment
         work = self.new_state()
-        q = self._adj_desc(adj, tmp, acc)
+        g_src = torch.zeros(len(self.shots), dtype=torch.float32, device=self.device)
         gw = self.desc(work)
         for s, shot in enumerate(self.shots):
+            q = self._adj_desc(adj, tmp, acc, g_src[s:] if self.force else None)
             res = grad_data[s].detach().to(torch.float32).contiguous()
             rres = self.receivers(res)
             src = self.source(shot)
@@ -276,10 +292,13 @@ class Elastic2DModeling:
         L.check(lib.sw_ssg2d_model_gradient(self.desc(work), q, L.fptr(self.vp_d), L.fptr(self.vs_d),
                                             L.fptr(out[0]), L.fptr(out[1]), L.fptr(out[2]), st),
                 "sw_ssg2d_model_gradient")
+        self._force_gradient(out[2], g_src)
         return tuple(out[i].t() for i in range(3))
 
-    def _adj_desc(self, adj, tmp, acc) -> "L.SwAdj2D":
+    def _adj_desc(self, adj, tmp, acc, g_src=None) -> "L.SwAdj2D":
         q = L.SwAdj2D()
+        if g_src is not None:
+            q.g_src = L.fptr(g_src)
         plane = 4 * self.nx * self.nz
         base = adj.data_ptr()
         for name, row in _FIELD_ROW.items():

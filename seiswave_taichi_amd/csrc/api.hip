@@ -180,6 +180,20 @@ __global__ void k_inject2d_batch(sw_ssg2d g, sw_source s, ShotCells sc, int it, 
     inject2d(g, s, it);
 }
 
+// Adjoint of a force dt*amp[it]/rho[p] added to a velocity between the
+// sweeps: the state's adjoint passes through unchanged, and d misfit /
+// d amp[it] is the adjoint of that velocity at p, accumulated here times
+// amp[it] (one block per shot, steps in a fixed order).
+__global__ void k_adj_force_batch(sw_adj2d q, sw_source s, ShotCells sc, int it, long as, int nx)
+{
+    const int b = blockIdx.x;
+    const float *lam = (s.kind == SW_SRC_FORCE_VZ ? q.vz : q.vx) + (long)b * as;
+    q.g_src[b] += s.amp[it] * lam[(long)sc.iz[b] * nx + sc.ix[b]];
+}
+
+inline bool force_source(const sw_source *s) { return s->kind == SW_SRC_FORCE_VZ || s->kind == SW_SRC_FORCE_VX; }
+inline bool stress_source(const sw_source *s) { return s->kind == SW_SRC_MOMENT || s->kind == SW_SRC_EXPLOSIVE; }
+
 // Receivers of shot blockIdx.y: state at + y * bs, traces at + y * ds.
 __global__ void k_record2d_batch(sw_ssg2d g, sw_receivers r, int it, long bs, long ds)
 {
@@ -557,8 +571,9 @@ int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_a
     SW_REQUIRE(src && residual && work, "null source, residual or work buffer");
     SW_REQUIRE(n >= 0 && it0 >= 0, "bad step range");
     if (n == 0) return SW_OK;
-    SW_REQUIRE(src->kind == SW_SRC_NONE || src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE,
-               "the adjoint covers stress sources (explosive or moment tensor)");
+    SW_REQUIRE(src->kind == SW_SRC_NONE || stress_source(src) || force_source(src),
+               "the adjoint covers explosive, moment-tensor and force sources");
+    SW_REQUIRE(!force_source(src) || adj->g_src, "force sources need the adj->g_src accumulator");
     if ((rc = check_source2d(g, src, it0)) || (rc = check_source2d(g, src, it0 + n - 1))) return rc;
     if ((rc = check_receivers(residual, it0)) || (rc = check_receivers(residual, it0 + n - 1))) return rc;
     SW_REQUIRE(residual->ncomp <= 16, "too many residual components");
@@ -576,10 +591,14 @@ int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_a
         e.mem = base + 5 * P;
         return e;
     };
+    ShotCells sc{};
+    sc.ix[0] = src->ix;
+    sc.iz[0] = src->iz;
     for (int t = 0; t < n; ++t) {
-        if (src->kind != SW_SRC_NONE) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it0 + t);
+        if (stress_source(src)) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it0 + t);
         SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t) * S, g->vx, S * sizeof(float), hipMemcpyDeviceToDevice, st));
         if ((rc = launch_ssg2d_velocity(g, st))) return rc;
+        if (force_source(src)) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it0 + t);
         SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t + 1) * S, g->vx, S * sizeof(float), hipMemcpyDeviceToDevice,
                                   st));
         if ((rc = launch_ssg2d_stress(g, st))) return rc;
@@ -588,6 +607,8 @@ int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_a
         if (residual->nrec && (rc = launch_adj2d_inject(adj, residual, it0 + t, st))) return rc;
         const sw_ssg2d mid = state_at(work + (2 * (size_t)t + 1) * S);
         if ((rc = launch_adj2d_stress(&mid, adj, st))) return rc;
+        if (force_source(src))
+            hipLaunchKernelGGL(k_adj_force_batch, dim3(1), dim3(1), 0, st, *adj, *src, sc, it0 + t, 0L, g->nx);
         const sw_ssg2d pre = state_at(work + (2 * (size_t)t) * S);
         if ((rc = launch_adj2d_velocity(&pre, adj, st))) return rc;
     }
@@ -608,8 +629,8 @@ static int check_batch2d(const sw_ssg2d *g, int nbatch, long state_stride, const
     SW_REQUIRE(g->vz == g->vx + P && g->sxx == g->vx + 2 * P && g->szz == g->vx + 3 * P &&
                    g->sxz == g->vx + 4 * P && g->mem == g->vx + 5 * P,
                "the state must be one block of 15 planes (vx vz sxx szz sxz mem) starting at vx");
-    SW_REQUIRE(src->kind == SW_SRC_NONE || src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE,
-               "shot batches take stress sources (explosive or moment tensor)");
+    SW_REQUIRE(src->kind == SW_SRC_NONE || stress_source(src) || force_source(src),
+               "shot batches take explosive, moment-tensor and force sources");
     for (int b = 0; b < nbatch; ++b) {
         const int ix = cells[2 * b], iz = cells[2 * b + 1];
         SW_REQUIRE(ix >= 1 && ix < g->nx - 1 && iz >= 1 && iz < g->nz - 1, "shot %d cell (%d,%d) outside the grid", b,
@@ -645,9 +666,11 @@ int sw_ssg2d_run_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, 
     if (rec && (rc = check_batch_receivers(g, rec, it0, nsteps))) return rc;
     hipStream_t st = as_stream(stream);
     for (int it = it0; it < it0 + nsteps; ++it) {
-        if (src->kind != SW_SRC_NONE)
+        if (stress_source(src))
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, (long)state_stride);
         if ((rc = launch_ssg2d_velocity_batch(g, nbatch, state_stride, st))) return rc;
+        if (force_source(src))
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, (long)state_stride);
         if ((rc = launch_ssg2d_stress_batch(g, nbatch, state_stride, st))) return rc;
         if (rec && rec->nrec) {
             const int n = rec->nrec * rec->ncomp;
@@ -674,6 +697,7 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
         SW_REQUIRE(src->amp != nullptr && it0 + n <= src->namp, "source amplitudes missing for the steps");
     }
     if ((rc = check_batch_receivers(g, residual, it0, n))) return rc;
+    SW_REQUIRE(!force_source(src) || adj->g_src, "force sources need the adj->g_src accumulator");
     const long P = (long)g->nx * g->nz, S = 15 * P;
     SW_REQUIRE(nbatch == 1 || adj_stride >= 24 * P, "adjoint stride %ld below one adjoint block (%ld floats)",
                (long)adj_stride, 24 * P);
@@ -687,11 +711,13 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
         return e;
     };
     for (int t = 0; t < n; ++t) {
-        if (src->kind != SW_SRC_NONE)
+        if (stress_source(src))
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
         SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t) * slot, g->vx, slot * sizeof(float), hipMemcpyDeviceToDevice,
                                   st));
         if ((rc = launch_ssg2d_velocity_batch(g, nbatch, bs, st))) return rc;
+        if (force_source(src))
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
         SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t + 1) * slot, g->vx, slot * sizeof(float),
                                   hipMemcpyDeviceToDevice, st));
         if ((rc = launch_ssg2d_stress_batch(g, nbatch, bs, st))) return rc;
@@ -701,6 +727,8 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
             return rc;
         const sw_ssg2d mid = state_at(work + (2 * (size_t)t + 1) * slot);
         if ((rc = launch_adj2d_stress_batch(&mid, adj, nbatch, bs, as, st))) return rc;
+        if (force_source(src))
+            hipLaunchKernelGGL(k_adj_force_batch, dim3(nbatch), dim3(1), 0, st, *adj, *src, sc, it0 + t, as, g->nx);
         const sw_ssg2d pre = state_at(work + (2 * (size_t)t) * slot);
         if ((rc = launch_adj2d_velocity_batch(&pre, adj, nbatch, bs, as, st))) return rc;
     }

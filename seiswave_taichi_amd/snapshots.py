@@ -39,11 +39,19 @@ class Snapshots:
         self.frames: list[tuple[int, dict]] = []
 
     def _views(self):
+        """Device tensors to stage: whole fields in their device layout (a
+        plain contiguous copy; the host side reverses the axes), or the index
+        of the reference-order view."""
         out = {}
         for n in self.fields:
-            v = getattr(self.sim, n).to_torch()
-            out[n] = v if self.index is None else v[self.index]
+            f = getattr(self.sim, n)
+            out[n] = f.data if self.index is None else f.to_torch()[self.index]
         return out
+
+    def _host(self, t: torch.Tensor) -> np.ndarray:
+        a = t.numpy()
+        # whole fields: the reference (x, y, z) order as a view, no host copy
+        return a.transpose(*reversed(range(a.ndim))) if self.index is None else a
 
     def _advance(self, it0: int, n: int):
         if n <= 0:
@@ -89,4 +97,4 @@ class Snapshots:
             self._take(nxt)
             it = nxt + 1
         self.copy_stream.synchronize()
-        return [(k, {n: t.numpy() for n, t in h.items()}) for k, h in self.frames]
+        return [(k, {n: self._host(t) for n, t in h.items()}) for k, h in self.frames]

@@ -189,3 +189,102 @@ def test_shot_batch_matches_shot_by_shot():
         grads.append([t.grad.double().cpu().numpy() for t in tv])
     for a, b in zip(*grads):
         assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b), np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _oracle_misfit_force(vp, vs, rho, m, rec, p, dobs, shots, amps):
+    """f64 oracle forward of the VS step with its own vz force
+    (vz[p] += dt*amp/rho[p] after the velocity loops); returns (J, data)."""
+    total = 0.0
+    datas = []
+    for s, (ix, iz) in enumerate(shots):
+        o = O.Oracle2D(vs, vp, rho, m.dx, m.dz, m.dt, ix, iz, 0, 0, m.nt, m.order, freq=20.0, precision="f64")
+        o.set_pml([True] * 4, p)
+        data = np.zeros((m.nt, 2, len(rec)))
+        for it in range(m.nt):
+            o.vs_step(it, float(amps[it]))
+            data[it, 0] = o.fields["vx"][rec[:, 0], rec[:, 1]]
+            data[it, 1] = o.fields["vz"][rec[:, 0], rec[:, 1]]
+        datas.append(data)
+        if dobs is not None:
+            total += 0.5 * float(np.sum((data - dobs[s]) ** 2))
+    return total, np.stack(datas)
+
+
+def test_force_source_gradient_matches_finite_differences():
+    """A vz force (dt*amp/rho at the source, after the velocity sweep): the
+    adjoint gradient against central differences of the f64 oracle at 12
+    cells per parameter (<= 1e-3), and d/d rho at the source cell itself,
+    where the force's own -dt*amp/rho**2 term enters (<= 1e-2)."""
+    from seiswave_taichi_amd.adjoint import Elastic2DModeling, seismograms
+    shape, d, nt = (60, 48), (10.0, 10.0), 160
+    rx = np.arange(8, shape[0] - 8, 4)
+    rec = np.stack([rx, np.full_like(rx, 5)], 1)
+    p = pml_dict(4700.0, 8, d, 20.0, 2700.0, dims=2)
+    shots = ((30, 14),)
+    amps = 1e6 * ricker2(np.arange(nt), 8e-4, 1.5 / 20.0, 20.0, 1.0)
+    m = Elastic2DModeling(shape, *d, 8e-4, nt, 2, p, shots, rec, freq=20.0, checkpoint=40, wavelet=amps,
+                          source="force_vz")
+    vs, vp, rho = layered_model(shape, seed=14)
+    ii, jj = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    bump = np.exp(-((ii - 30) ** 2 + (jj - 24) ** 2) / (2 * 4.0 ** 2))
+    _, dobs = _oracle_misfit_force(vp.astype(np.float64) * (1 + 0.08 * bump), vs.astype(np.float64),
+                                   rho.astype(np.float64) * (1 - 0.05 * bump), m, rec, p, None, shots, amps)
+    data, _ = m.model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
+    _, ref = _oracle_misfit_force(vp.astype(np.float64), vs.astype(np.float64), rho.astype(np.float64), m, rec,
+                                  p, None, shots, amps)
+    assert np.linalg.norm(data.cpu().numpy() - ref) <= 1e-5 * np.linalg.norm(ref)
+    tv = [torch.from_numpy(a).to("cuda").requires_grad_(True) for a in (vp, vs, rho)]
+    loss = 0.5 * torch.sum((seismograms(*tv, m) - torch.from_numpy(dobs.astype(np.float32)).cuda()) ** 2)
+    loss.backward()
+    grads = [t.grad.cpu().numpy().astype(np.float64) for t in tv]
+    base = [vp.astype(np.float64), vs.astype(np.float64), rho.astype(np.float64)]
+
+    def fd(k, ci, cj):
+        h = 1e-4 * base[k][ci, cj]
+        mp = [b.copy() for b in base]
+        mm = [b.copy() for b in base]
+        mp[k][ci, cj] += h
+        mm[k][ci, cj] -= h
+        jp, _ = _oracle_misfit_force(*mp, m, rec, p, dobs, shots, amps)
+        jm, _ = _oracle_misfit_force(*mm, m, rec, p, dobs, shots, amps)
+        return (jp - jm) / (2 * h)
+
+    rng = np.random.default_rng(3)
+    cand = [(a, b) for a in range(12, shape[0] - 12) for b in range(12, shape[1] - 12)
+            if max(abs(a - 30), abs(b - 14)) >= 4]
+    cells = [cand[t] for t in rng.choice(len(cand), 12, replace=False)]
+    for k, name in enumerate(("vp", "vs", "rho")):
+        f = np.array([fd(k, ci, cj) for ci, cj in cells])
+        a = np.array([grads[k][ci, cj] for ci, cj in cells])
+        assert np.linalg.norm(a - f) <= 1e-3 * np.linalg.norm(f), (name, np.linalg.norm(a - f) / np.linalg.norm(f))
+    f_src = fd(2, 30, 14)
+    assert abs(grads[2][30, 14] - f_src) <= 1e-2 * abs(f_src), (grads[2][30, 14], f_src)
+
+
+@pytest.mark.parametrize("source", ["force_vz", "force_vx"])
+def test_force_source_batch_matches_shot_by_shot(source):
+    from seiswave_taichi_amd.adjoint import seismograms
+    shots = ((30, 14), (18, 20), (44, 16))
+    amps = 1e6 * ricker2(np.arange(160), 8e-4, 1.5 / 20.0, 20.0, 1.0)
+    mk = lambda: _modeling(shots=shots, ck=30)  # noqa: E731
+    mb, _, _ = mk()
+    ms, _, _ = mk()
+    for m in (mb, ms):
+        m.src_kind = {"force_vz": 1, "force_vx": 4}[source]
+        m.force = True
+        m.amp = torch.from_numpy(amps.astype(np.float32)).cuda()
+    ms.batch = False
+    vs, vp, rho = layered_model((mb.nx, mb.nz), seed=15)
+    db, _ = mb.model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
+    ds, _ = ms.model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
+    np.testing.assert_array_equal(db.cpu().numpy(), ds.cpu().numpy())
+    target = (0.9 * db).detach()
+    grads = []
+    for m in (mb, ms):
+        tv = [torch.from_numpy(a).to("cuda").requires_grad_(True) for a in (vp, vs, rho)]
+        loss = 0.5 * torch.sum((seismograms(*tv, m) - target) ** 2)
+        loss.backward()
+        grads.append([t.grad.double().cpu().numpy() for t in tv])
+    for a, b in zip(*grads):
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    assert np.abs(grads[0][2][30, 14]) > 0
