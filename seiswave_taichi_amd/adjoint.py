@@ -9,8 +9,9 @@ of the seismograms with respect to vp, vs and rho.
 
 Memory: the forward keeps a checkpoint of the full state every
 ``checkpoint`` steps; the backward recomputes each segment forward from its
-checkpoint, keeping the state before and after the velocity stage of every
-step of the segment, then runs the adjoint backwards through it, one native
+checkpoint, keeping of every step the 15 planes the adjoint stages read
+(stresses and their memory variables before the velocity stage, velocities
+and theirs after it), then runs the adjoint backwards through it, one native
 call per segment (sw_ssg2d_adjoint_segment).
 
 Sources: explosive or moment tensor (added to the stresses before the
@@ -233,7 +234,7 @@ class Elastic2DModeling:
             q = self._adj_desc(adjb[0, :NSTATE], adjb[0, NSTATE:NSTATE + 4], adjb[0, NSTATE + 4:],
                                g_src[group[0]:] if self.force else None)
             work = self.new_state(nb)
-            seg = self.new_state(2 * K * nb)
+            seg = self.new_state(K * nb)  # 15 planes per shot and step: what the adjoint stages read
             g = self.desc(work[0])
             src = self.source(self.shots[group[0]])
             res = grad_data[group[0]:group[-1] + 1].detach().to(torch.float32).contiguous()
@@ -273,7 +274,7 @@ class Elastic2DModeling:
         acc = torch.zeros((5, nz, nx), dtype=torch.float32, device=self.device)  # g_b1 g_b4 g_lamh g_muh g_muz
         adj = self.new_state()
         tmp = torch.zeros((4, nz, nx), dtype=torch.float32, device=self.device)
-        seg = self.new_state(2 * K)  # states before / after the velocity sweep of each step of a segment
+        seg = self.new_state(K)  # per step: the planes the adjoint stages read (before / after the velocity sweep)
         work = self.new_state()
         g_src = torch.zeros(len(self.shots), dtype=torch.float32, device=self.device)
         gw = self.desc(work)

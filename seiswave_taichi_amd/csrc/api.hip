@@ -566,53 +566,10 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
 int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_adj2d *adj,
                              const sw_receivers *residual, float *work, int32_t it0, int32_t n, void *stream)
 {
-    int rc = check_adj2d(g, adj);
-    if (rc) return rc;
-    SW_REQUIRE(src && residual && work, "null source, residual or work buffer");
-    SW_REQUIRE(n >= 0 && it0 >= 0, "bad step range");
-    if (n == 0) return SW_OK;
-    SW_REQUIRE(src->kind == SW_SRC_NONE || stress_source(src) || force_source(src),
-               "the adjoint covers explosive, moment-tensor and force sources");
-    SW_REQUIRE(!force_source(src) || adj->g_src, "force sources need the adj->g_src accumulator");
-    if ((rc = check_source2d(g, src, it0)) || (rc = check_source2d(g, src, it0 + n - 1))) return rc;
-    if ((rc = check_receivers(residual, it0)) || (rc = check_receivers(residual, it0 + n - 1))) return rc;
-    SW_REQUIRE(residual->ncomp <= 16, "too many residual components");
-    for (int c = 0; c < residual->ncomp; ++c)
-        SW_REQUIRE(fields2d(g).f[residual->comp[c]] != nullptr, "component %d does not exist in 2-D",
-                   residual->comp[c]);
-    const size_t P = (size_t)g->nx * g->nz, S = 15 * P;
-    SW_REQUIRE(g->vz == g->vx + P && g->sxx == g->vx + 2 * P && g->szz == g->vx + 3 * P &&
-                   g->sxz == g->vx + 4 * P && g->mem == g->vx + 5 * P,
-               "the state must be one block of 15 planes (vx vz sxx szz sxz mem) starting at vx");
-    hipStream_t st = as_stream(stream);
-    auto state_at = [&](float *base) {
-        sw_ssg2d e = *g;
-        e.vx = base; e.vz = base + P; e.sxx = base + 2 * P; e.szz = base + 3 * P; e.sxz = base + 4 * P;
-        e.mem = base + 5 * P;
-        return e;
-    };
-    ShotCells sc{};
-    sc.ix[0] = src->ix;
-    sc.iz[0] = src->iz;
-    for (int t = 0; t < n; ++t) {
-        if (stress_source(src)) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it0 + t);
-        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t) * S, g->vx, S * sizeof(float), hipMemcpyDeviceToDevice, st));
-        if ((rc = launch_ssg2d_velocity(g, st))) return rc;
-        if (force_source(src)) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it0 + t);
-        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t + 1) * S, g->vx, S * sizeof(float), hipMemcpyDeviceToDevice,
-                                  st));
-        if ((rc = launch_ssg2d_stress(g, st))) return rc;
-    }
-    for (int t = n - 1; t >= 0; --t) {
-        if (residual->nrec && (rc = launch_adj2d_inject(adj, residual, it0 + t, st))) return rc;
-        const sw_ssg2d mid = state_at(work + (2 * (size_t)t + 1) * S);
-        if ((rc = launch_adj2d_stress(&mid, adj, st))) return rc;
-        if (force_source(src))
-            hipLaunchKernelGGL(k_adj_force_batch, dim3(1), dim3(1), 0, st, *adj, *src, sc, it0 + t, 0L, g->nx);
-        const sw_ssg2d pre = state_at(work + (2 * (size_t)t) * S);
-        if ((rc = launch_adj2d_velocity(&pre, adj, st))) return rc;
-    }
-    return launch_status("ssg2d adjoint segment");
+    SW_REQUIRE(g && src, "null descriptor or source");
+    const int32_t cell[2] = {src->ix, src->iz};
+    return sw_ssg2d_adjoint_segment_batch(g, 1, 15L * g->nx * g->nz, src, cell, adj, 0, residual, 0, work, it0, n,
+                                          stream);
 }
 
 // ---- shot batches of the 2-D VS path ----
@@ -702,35 +659,49 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
     SW_REQUIRE(nbatch == 1 || adj_stride >= 24 * P, "adjoint stride %ld below one adjoint block (%ld floats)",
                (long)adj_stride, 24 * P);
     const long bs = nbatch == 1 ? 0 : (long)state_stride, as = nbatch == 1 ? 0 : (long)adj_stride;
-    const long slot = (long)nbatch * (nbatch == 1 ? S : (long)state_stride);  // one saved batch state
+    // Kept per step, per shot: what the two adjoint stages read of the forward
+    // state.  The velocity stage reads sxx szz sxz and memory variables 0..5
+    // (state planes 2..10, before the velocity sweep); the stress stage reads
+    // vx vz and memory variables 6..9 (planes 0..1 and 11..14, after it).
+    const long pre_n = 9 * P, mid_n = 6 * P;
+    const long step_n = (long)nbatch * (pre_n + mid_n);
+    const size_t pitch = sizeof(float) * (size_t)(nbatch == 1 ? S : (long)state_stride);
     hipStream_t st = as_stream(stream);
-    auto state_at = [&](float *base) {
-        sw_ssg2d e = *g;
-        e.vx = base; e.vz = base + P; e.sxx = base + 2 * P; e.szz = base + 3 * P; e.sxz = base + 4 * P;
-        e.mem = base + 5 * P;
-        return e;
+    auto rows = [&](float *dst, long dst_pitch, const float *src0, long width) -> hipError_t {
+        if (nbatch == 1)
+            return hipMemcpyAsync(dst, src0, sizeof(float) * width, hipMemcpyDeviceToDevice, st);
+        return hipMemcpy2DAsync(dst, sizeof(float) * dst_pitch, src0, pitch, sizeof(float) * width, nbatch,
+                                hipMemcpyDeviceToDevice, st);
     };
     for (int t = 0; t < n; ++t) {
+        float *pre = work + (size_t)t * step_n, *mid = pre + (size_t)nbatch * pre_n;
         if (stress_source(src))
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
-        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t) * slot, g->vx, slot * sizeof(float), hipMemcpyDeviceToDevice,
-                                  st));
+        SW_TRY_HIP(rows(pre, pre_n, g->vx + 2 * P, pre_n));
         if ((rc = launch_ssg2d_velocity_batch(g, nbatch, bs, st))) return rc;
         if (force_source(src))
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
-        SW_TRY_HIP(hipMemcpyAsync(work + (2 * (size_t)t + 1) * slot, g->vx, slot * sizeof(float),
-                                  hipMemcpyDeviceToDevice, st));
+        SW_TRY_HIP(rows(mid, mid_n, g->vx, 2 * P));
+        SW_TRY_HIP(rows(mid + 2 * P, mid_n, g->vx + 11 * P, 4 * P));
         if ((rc = launch_ssg2d_stress_batch(g, nbatch, bs, st))) return rc;
     }
     for (int t = n - 1; t >= 0; --t) {
+        float *pre = work + (size_t)t * step_n, *mid = pre + (size_t)nbatch * pre_n;
         if (residual->nrec && (rc = launch_adj2d_inject_batch(adj, residual, it0 + t, nbatch, as, data_stride, st)))
             return rc;
-        const sw_ssg2d mid = state_at(work + (2 * (size_t)t + 1) * slot);
-        if ((rc = launch_adj2d_stress_batch(&mid, adj, nbatch, bs, as, st))) return rc;
+        // fields a stage does not read point at valid memory of the same step
+        sw_ssg2d fm = *g;
+        fm.vx = mid; fm.vz = mid + P;
+        fm.mem = mid - 4 * P;  // memory variables 6..9 at mid + 2P .. mid + 6P
+        fm.sxx = fm.szz = fm.sxz = mid;
+        if ((rc = launch_adj2d_stress_batch(&fm, adj, nbatch, mid_n, as, st))) return rc;
         if (force_source(src))
             hipLaunchKernelGGL(k_adj_force_batch, dim3(nbatch), dim3(1), 0, st, *adj, *src, sc, it0 + t, as, g->nx);
-        const sw_ssg2d pre = state_at(work + (2 * (size_t)t) * slot);
-        if ((rc = launch_adj2d_velocity_batch(&pre, adj, nbatch, bs, as, st))) return rc;
+        sw_ssg2d fp = *g;
+        fp.sxx = pre; fp.szz = pre + P; fp.sxz = pre + 2 * P;
+        fp.mem = pre + 3 * P;  // memory variables 0..5
+        fp.vx = fp.vz = pre;
+        if ((rc = launch_adj2d_velocity_batch(&fp, adj, nbatch, pre_n, as, st))) return rc;
     }
     return launch_status("ssg2d adjoint segment batch");
 }
