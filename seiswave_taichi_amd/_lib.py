@@ -130,6 +130,9 @@ def load(path: str | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
+    # torch first: the process must run one HIP runtime, and torch's is the one
+    # that allocates the device memory this library is handed
+    import torch  # noqa: F401
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(

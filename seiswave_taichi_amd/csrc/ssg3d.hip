@@ -36,6 +36,13 @@
 #ifndef SW_PREFETCH
 #define SW_PREFETCH 1
 #endif
+// SW_SPLIT_PREFETCH=1 also fetches the velocity split-field values of plane
+// k+1 with the plane (split-field PML class; 4 % on that sweep); 0 loads them
+// when plane k is computed.  The stress sweep loads its 15 when computing: a
+// prefetch there measured no gain.
+#ifndef SW_SPLIT_PREFETCH
+#define SW_SPLIT_PREFETCH 1
+#endif
 
 namespace sw {
 
@@ -102,6 +109,22 @@ __device__ __forceinline__ SpCol spml_col(const sw_ssg3d &g, int i, int j)
     return c;
 }
 
+// The split values q[t] = split[first + t][cell] (t < M) of a cell whose
+// integer or half point lies in the PML on some axis at plane k; the others
+// are not loaded (their plain update needs none).
+template <int M, int PM>
+__device__ __forceinline__ void load_split(float *q, const sw_ssg3d &g, const SpCol &sp, long cell, int k, int first)
+{
+    const bool zi = (PM & 4) ? g.pz.in[k] != 0 : false;
+    const bool zh = (PM & 4) ? g.pz.in_h[k] != 0 : false;
+    if (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh) {
+        const long NC = (long)g.nx * g.ny * g.nz;
+        const float *qp = g.split + cell;
+#pragma unroll
+        for (int t = 0; t < M; ++t) q[t] = qp[(long)(first + t) * NC];
+    }
+}
+
 // Crank-Nicolson step of one split component: q = (n q + incr) / d
 // (src/Elastic3D_SSG_SPML.py:199-204 and alike); q0 is the value at dst.
 __device__ __forceinline__ float cn_at(float q0, float *dst, float n, float d, float incr)
@@ -149,13 +172,18 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
         wzz[t] = ld(R.szz, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
     }
     VPlane<N> P;
+    const long cell0 = (long)j * nx + i, plane = (long)nx * ny;
 #if SW_PREFETCH
     load_vplane<N, CPM>(P, g, R, pc, k0, SY, SZ, sl);
+    if (SW_SPLIT_PREFETCH && SPLIT) load_split<9, PM>(P.q, g, sp, cell0 + k0 * plane, k0, 0);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         VPlane<N> Q;
-        if (k + 1 < cl.k1) load_vplane<N, CPM>(Q, g, R, pc, k + 1, SY, SZ, sl);
+        if (k + 1 < cl.k1) {
+            load_vplane<N, CPM>(Q, g, R, pc, k + 1, SY, SZ, sl);
+            if (SW_SPLIT_PREFETCH && SPLIT) load_split<9, PM>(Q.q, g, sp, cell0 + (k + 1) * plane, k + 1, 0);
+        }
 #else
         load_vplane<N, CPM>(P, g, R, pc, k, SY, SZ, sl);
 #endif
@@ -168,10 +196,14 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
         // all split values of this cell first, so their loads overlap
         float qv[9];
         const bool any_split = SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh);
+#if SW_PREFETCH && SW_SPLIT_PREFETCH
+        if (SPLIT) copy_n<9>(qv, P.q);
+#else
         if (any_split) {
 #pragma unroll
             for (int t = 0; t < 9; ++t) qv[t] = qp[(long)t * NC];
         }
+#endif
         const uint32_t p = pc + (uint32_t)k * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
         const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;

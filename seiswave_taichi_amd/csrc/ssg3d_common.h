@@ -210,6 +210,7 @@ struct VPlane {
     float v[3];
     float wxz, wyz, wzz; // next window entries: sxz, syz at k+N-1, szz at k+N
     float mx[3], my[3], mz[3];
+    float q[9];          // split-field PML: the 9 velocity split values of the cell
     Coef cz;
     int zs;
 };
