@@ -208,13 +208,51 @@ def gradient_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def vs2d_workload(args):
+    """2-D VS scheme at 400 x 400 (10 m, vp 3000, vs 1732, rho 2000, dt 1 ms,
+    accuracy 2, 20-cell CPML on four sides), explosive 15 Hz Ricker2 at
+    (200, 200), 200 receivers at j = 7 recording vx and vz: time per step of a
+    1000-step run (13 MB per step, cache resident: not an HBM case)."""
+    import torch
+
+    from seiswave_taichi_amd.Elastic2D_CPML import ElasticWAVE
+    from seiswave_taichi_amd.coeffs import ricker2
+
+    torch.cuda.set_device(0)
+    n, f, dt = 400, 15.0, 1e-3
+    nt = args.warmup + args.steps
+    vs, vp, rho = (np.full((n, n), v, np.float32) for v in (1732.0, 3000.0, 2000.0))
+    rx = np.round(np.linspace(3, n - 4, 200)).astype(np.int64)
+    amps = ricker2(np.arange(nt), dt, 1.5 / f, f, 1.0)
+    w = ElasticWAVE(vs, vp, rho, 10.0, 10.0, dt, 200, 200, 0, 0, nt, 2, f,
+                    receivers=np.stack([rx, np.full_like(rx, 7)], 1), components=("vx", "vz"),
+                    source={"kind": "explosive", "cell": (200, 200), "wavelet": amps})
+    w.SetADEPML2D([True] * 4, {"vp_max": 3000.0, "pml_x_thick": 200.0, "pml_z_thick": 200.0, "Rcoef": 1e-6,
+                               "theta": 1, "alpha_max_pml": np.pi * f, "kmax_pml": 1732.0 / (5 * 10.0 * f)})
+    w.run("VS", args.warmup, 0)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    w.run("VS", args.steps, args.warmup)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    if not np.all(np.isfinite(w.data.to_numpy())):
+        raise RuntimeError("non-finite seismogram")
+    out = {"metric": "2D VS ms/step", "value": round(ms / args.steps, 5), "unit": "ms/step",
+           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f32",
+           "data": "synthetic", "Gcells_per_s": round(n * n * args.steps / (ms / 1e3) / 1e9, 3),
+           "config": {"workload": "vs2d_400: 400x400 2-D VS, explosive 15 Hz, 200 receivers (vx, vz), CPML20"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="tensor_200",
-                    choices=["tensor_200", "spml_200", "layered_600x600x400", "gradient_800x300"])
+                    choices=["tensor_200", "spml_200", "layered_600x600x400", "gradient_800x300", "vs2d_400"])
     ap.add_argument("--fd-cells", type=int, default=4,
                     help="gradient_800x300: cells per parameter of the finite-difference spot check")
     ap.add_argument("--kernel-reps", type=int, default=200)
@@ -230,6 +268,8 @@ def main():
 
     if args.config == "gradient_800x300":
         return gradient_workload(args)
+    if args.config == "vs2d_400":
+        return vs2d_workload(args)
 
     import torch
 

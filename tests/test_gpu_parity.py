@@ -390,3 +390,48 @@ def test_full_size_600x600x400_matches_oracle():
     np.testing.assert_array_equal(w.data.to_numpy().reshape(o.data.shape), o.data)
     del w, o
     torch.cuda.empty_cache()
+
+
+def _config1(kind):
+    """400 x 400, 10 m, vp 3000, vs 1732, rho 2000, dt 1 ms, 1000 steps,
+    accuracy 2, 20-cell CPML on four sides (alpha_max 15 pi, kmax 2.309), 15 Hz
+    Ricker2 source at (200, 200), 200 receivers at j = 7 recording vx and vz."""
+    from seiswave_taichi_amd.Elastic2D_CPML import ElasticWAVE
+    n, steps, f, dt, d = 400, 1000, 15.0, 1e-3, (10.0, 10.0)
+    vs, vp, rho = homogeneous((n, n), vp=3000.0, vs=1732.0, rho=2000.0)
+    rx = np.round(np.linspace(3, n - 4, 200)).astype(np.int64)
+    rec = np.stack([rx, np.full_like(rx, 7)], 1)
+    p = {"vp_max": 3000.0, "pml_x_thick": 200.0, "pml_z_thick": 200.0, "Rcoef": 1e-6, "theta": 1,
+         "alpha_max_pml": np.pi * 15.0, "kmax_pml": 1732.0 / (5 * 10.0 * 15.0)}
+    amps = ricker2(np.arange(steps), dt, 1.5 / f, f, 1.0)
+    w = ElasticWAVE(vs, vp, rho, *d, dt, 200, 200, 0, 0, steps, 2, f, receivers=rec, components=("vx", "vz"),
+                    source={"kind": kind, "cell": (200, 200), "wavelet": amps})
+    w.SetADEPML2D([True] * 4, p)
+    w.run("VS", steps, 0)
+    o = O.Oracle2D(vs, vp, rho, *d, dt, 200, 200, 0, 0, steps, 2, f)
+    o.set_pml([True] * 4, p)
+    ref = np.zeros((steps, 2, len(rx)), np.float32)
+    for it in range(steps):
+        if kind == "explosive":  # sxx, szz += amp*(-1/sqrt(2)) before the velocity loops
+            e = np.float32(amps[it]) * np.float32(-0.70710678118654752)
+            o.fields["sxx"][200, 200] += e
+            o.fields["szz"][200, 200] += e
+            o.vs_step(it, 0.0)
+        else:
+            o.vs_step(it, float(amps[it]))
+        ref[it, 0] = o.fields["vx"][rx, 7]
+        ref[it, 1] = o.fields["vz"][rx, 7]
+    return w, o, ref
+
+
+@pytest.mark.parametrize("kind", ["force_vz", "explosive"])
+def test_config1_full_size_matches_oracle(kind):
+    """The 2-D parity configuration at its full size and length: the
+    reference-compatible vz force and the explosive source, 200 two-component
+    receivers, every field and trace bit for bit."""
+    w, o, ref = _config1(kind)
+    got = w.data.to_numpy()
+    assert got.shape == ref.shape and np.abs(ref).max() > 0
+    np.testing.assert_array_equal(got, ref)
+    for name in o.FIELDS:
+        np.testing.assert_array_equal(getattr(w, name).to_numpy(), o.fields[name], err_msg=name)
