@@ -336,3 +336,46 @@ def test_pml_absorbs_energy():
         o.run(300)
         energies[on] = sum(float(np.sum(o.fields[f].astype(np.float64) ** 2)) for f in ("vx", "vy", "vz"))
     assert energies[True] < 0.05 * energies[False]
+
+
+def test_s_wave_arrival_time():
+    """2-D vertical force in a homogeneous medium: along the horizontal through
+    the source the far field is pure S, so vz there peaks near
+    t0 + r/vs (t0 = 1.5/f, the Ricker2 delay of update_SSG_VS)."""
+    n, vp0, vs0, f, dt = 120, 3000.0, 1732.0, 15.0, 1e-3
+    vs, vp, rho = homogeneous((n, n), vp=vp0, vs=vs0, rho=2000.0)
+    nt, r = 330, 20
+    o = O.Oracle2D(vs, vp, rho, 10.0, 10.0, dt, 50, 60, 50 + r, 60, nt, 2, freq=f, precision="f32")
+    o.set_pml([True] * 4, pml_dict(vp0, 20, (10.0, 10.0), f, vs0, dims=2))
+    for it in range(nt):
+        o.vs_step(it)
+    t_peak_obs = np.argmax(np.abs(o.data)) * dt
+    t_expect = 1.5 / f + r * 10.0 / vs0
+    assert abs(t_peak_obs - t_expect) < 1.0 / f, (t_peak_obs, t_expect)
+    # and not the P time: r/vp is 0.05 s earlier
+    assert abs(t_peak_obs - (1.5 / f + r * 10.0 / vp0)) > 0.5 * abs(t_peak_obs - t_expect) + 0.02
+
+
+def test_reciprocity_of_force_responses():
+    """Source-receiver reciprocity of the discrete 2-D VS step in a layered
+    medium (f64, no PML, before any boundary reflection returns): the vz
+    response at B to a vz force at A equals the response at A to the force at
+    B, once each is weighted by rho[src]/rho_vz[src] (the force adds
+    dt*amp/rho at the integer cell while the vz update uses the four-point
+    average at the vz point)."""
+    shape = (90, 80)
+    vs, vp, rho = layered_model(shape, seed=9)
+    vs, vp, rho = (a.astype(np.float64) for a in (vs, vp, rho))
+    a, b = (30, 33), (52, 45)
+    nt, dt = 120, 5e-4
+    traces = {}
+    for s, r in ((a, b), (b, a)):
+        o = O.Oracle2D(vs, vp, rho, 10.0, 10.0, dt, *s, *r, nt, 2, freq=25.0, precision="f64")
+        o.set_pml([False] * 4, pml_dict(float(vp.max()), 5, (10.0, 10.0), 25.0, float(vs.max()), dims=2))
+        for it in range(nt):
+            o.vs_step(it)
+        w = rho[s] / (0.25 * (rho[s] + rho[s[0], s[1] + 1] + rho[s[0] + 1, s[1] + 1] + rho[s[0] + 1, s[1]]))
+        traces[s] = o.data * w
+    ta, tb = traces[a], traces[b]
+    assert np.abs(ta).max() > 0
+    assert np.linalg.norm(ta - tb) <= 1e-10 * np.linalg.norm(ta), np.linalg.norm(ta - tb) / np.linalg.norm(ta)
