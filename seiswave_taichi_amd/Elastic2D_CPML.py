@@ -126,6 +126,11 @@ class ElasticWAVE:
     def _descriptor(self, scheme, need_pml=True):
         if self.pml is None and need_pml:
             raise RuntimeError("SetADEPML2D must be called before the first time step")
+        # built once per scheme and PML setting: the pointers are to tensors that
+        # live as long as the instance
+        cache = self.__dict__.setdefault("_desc_cache", {})
+        if self.pml is not None and scheme in cache:
+            return cache[scheme]
         g = L.SwSSG2D()
         g.nx, g.nz, g.order, g.scheme = self.nx, self.nz, self.star, scheme
         g.dx, g.dz, g.dt = float(self.dx), float(self.dz), float(self.dt)
@@ -144,6 +149,8 @@ class ElasticWAVE:
             a.lo, a.hi = 0, n
             setattr(g, name, a)
         g.mem = L.fptr(self.mem_t)
+        if self.pml is not None:
+            cache[scheme] = g
         return g
 
     def _stream(self):
@@ -168,6 +175,7 @@ class ElasticWAVE:
         az = axis_pml(self.nz, self.star, self.dz, self.dt, p["pml_z_thick"], p["vp_max"], p["Rcoef"],
                       p["alpha_max_pml"], p["kmax_pml"], p["theta"], pml_surface[2], pml_surface[3])
         self.pml = (ax, az)
+        self._desc_cache = {}
         self.pml_dev = []
         for a in (ax, az):
             coeff = torch.from_numpy(np.stack([a.a, a.b, a.k, a.a_h, a.b_h, a.k_h, a.d, a.d_h])).to(self.device)

@@ -91,6 +91,7 @@ class SSG3DEngine:
         shape = (self.nz, self.ny, self.nx)
         self.f = {n: torch.zeros(shape, dtype=torch.float32, device=dev) for n in WAVEFIELDS}
         self.pml = None
+        self._desc = None
         self._keep = []
 
     # ------------------------------------------------------------- PML
@@ -107,6 +108,7 @@ class SSG3DEngine:
             axes.append(axis_pml(n, self.star, d, self.dt, thick, p["vp_max"], p["Rcoef"],
                                  p["alpha_max_pml"], p["kmax_pml"], p["theta"], bool(lo), bool(hi)))
         self.pml = axes
+        self._desc = None
         dev = self.device
         self.pml_dev = []
         for ax in axes:
@@ -142,9 +144,13 @@ class SSG3DEngine:
 
     # ------------------------------------------------------- descriptor
     def descriptor(self) -> L.SwSSG3D:
+        """The sw_ssg3d of this state (built once per PML setting: every
+        pointer in it is to a tensor that lives as long as the engine)."""
         if self.pml is None:
             raise RuntimeError("SetADEPML3D must be called before the first time step "
                                "(the reference raises AttributeError on ymin_pml otherwise)")
+        if self._desc is not None:
+            return self._desc
         g = L.SwSSG3D()
         g.nx, g.ny, g.nz, g.order, g.cls = self.nx, self.ny, self.nz, self.star, self.cls
         g.dx, g.dy, g.dz, g.dt = float(self.dx), float(self.dy), float(self.dz), float(self.dt)
@@ -170,6 +176,7 @@ class SSG3DEngine:
             g.mem_x, g.mem_y, g.mem_z = (L.fptr(t) for t in self.mem)
         if self.split is not None:
             g.split = L.fptr(self.split)
+        self._desc = g
         return g
 
     def stream(self):
