@@ -146,3 +146,94 @@ def test_install_src_alias():
         for k in [k for k in sys.modules if k == "src" or k.startswith("src.")]:
             sys.modules.pop(k)
         sys.modules.update(saved)
+
+
+def _fake_3d(n=24, order=2):
+    """A descriptor that passes the host-side checks without any device memory
+    (the pointers are never dereferenced before the rejected argument)."""
+    import ctypes as C
+    g = L.SwSSG3D()
+    g.nx = g.ny = g.nz = n
+    g.order, g.cls = order, L.SW_CLASS_TENSOR
+    fake = C.cast(C.c_void_p(0x1000), L._fp)
+    for name in ("rho", "lam", "mu", "vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz",
+                 "mem_x", "mem_y", "mem_z"):
+        setattr(g, name, fake)
+    for ax in (g.px, g.py, g.pz):
+        for f in ("a", "b", "k", "a_h", "b_h", "k_h", "d", "d_h"):
+            setattr(ax, f, fake)
+        ax.lo, ax.hi = 6, n - 6
+    return g
+
+
+def test_run_flags_and_free_surface_arguments_are_checked():
+    import ctypes as C
+    lib = L.load()
+    g = _fake_3d()
+    src = L.SwSource()
+    src.kind = L.SW_SRC_NONE
+    assert lib.sw_ssg3d_run_ex(C.byref(g), C.byref(src), None, 0, 4, 0x10, None) == -1
+    assert "unknown run flags" in L.last_error()
+    g.order = g.nz  # the free-surface plane must lie inside the grid
+    assert lib.sw_ssg3d_free_surface(C.byref(g), None) == -1
+
+
+def test_force_adjoint_needs_its_accumulator():
+    import ctypes as C
+    lib = L.load()
+    nx, nz = 30, 26
+    P = nx * nz
+    base = 0x100000
+    g = L.SwSSG2D()
+    g.nx, g.nz, g.order, g.scheme = nx, nz, 2, L.SW_2D_VS
+    fp = lambda a: C.cast(C.c_void_p(a), L._fp)  # noqa: E731
+    g.rho = g.lam = g.mu = fp(base)
+    for r, name in enumerate(("vx", "vz", "sxx", "szz", "sxz")):
+        setattr(g, name, fp(base + 4 * r * P))
+    g.mem = fp(base + 4 * 5 * P)
+    for ax in (g.px, g.pz):
+        for f in ("a", "b", "k", "a_h", "b_h", "k_h", "d", "d_h"):
+            setattr(ax, f, fp(base))
+        ax.in_ = ax.in_h = C.cast(C.c_void_p(base), L._u8p)
+        ax.lo, ax.hi = 0, nx
+    g.pz.hi = nz
+    q = L.SwAdj2D()
+    for name in ("vx", "vz", "sxx", "szz", "sxz", "mem", "tmp", "g_b1", "g_b4", "g_lamh", "g_muh", "g_muz"):
+        setattr(q, name, fp(base))
+    src = L.SwSource()
+    src.kind, src.ix, src.iz = L.SW_SRC_FORCE_VZ, 10, 10
+    src.amp, src.namp = fp(base), 100
+    rec = L.SwReceivers()
+    rec.nrec, rec.ncomp, rec.nt = 0, 1, 100
+    rc = lib.sw_ssg2d_adjoint_segment(C.byref(g), C.byref(src), C.byref(q), C.byref(rec), fp(base), 0, 5, None)
+    assert rc == -1 and "g_src" in L.last_error()
+
+
+def test_header_struct_layouts_match_ctypes(tmp_path):
+    """sizeof and field offsets of every public struct, from the C compiler on
+    include/seiswave.h, against the ctypes mirror."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {"sw_axis": L.SwAxis, "sw_ssg3d": L.SwSSG3D, "sw_ssg3d_shadow": None, "sw_ssg2d": L.SwSSG2D,
+               "sw_adj2d": L.SwAdj2D, "sw_source": L.SwSource, "sw_receivers": L.SwReceivers, "sw_band": L.SwBand}
+    structs = {k: v for k, v in structs.items() if v is not None}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "seiswave.h"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in ct._fields_:
+            cf = "in" if fname == "in_" else fname
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                               check=True).stdout.splitlines())
+    for cname, ct in structs.items():
+        assert int(out[cname]) == C.sizeof(ct), cname
+        for fname, _ in ct._fields_:
+            assert int(out[f"{cname}.{fname}"]) == getattr(ct, fname).offset, f"{cname}.{fname}"
