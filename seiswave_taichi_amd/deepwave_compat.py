@@ -113,7 +113,7 @@ def elastic(lamb, mu, buoyancy, grid_spacing, dt: float, source_amplitudes_y=Non
             raise ValueError("source amplitudes [shot, source, nt] and locations [shot, source, 2] disagree")
     rec_y = _locations(receiver_locations_y, nshot, "receiver_locations_y")
     rec_x = _locations(receiver_locations_x, nshot, "receiver_locations_x")
-    vmax = float(vp.max()) if max_vel is None else float(max_vel)
+    vmax = float(vp.detach().max()) if max_vel is None else float(max_vel)
     courant = vmax * dt * np.sqrt(1.0 / dx ** 2 + 1.0 / dy ** 2) * float(np.sum(np.abs(diff_coff(order))))
     if courant >= 1.0:
         raise ValueError(f"dt = {dt} exceeds the Courant limit of this grid (number {courant:.3f} >= 1); "

@@ -435,3 +435,44 @@ def test_config1_full_size_matches_oracle(kind):
     np.testing.assert_array_equal(got, ref)
     for name in o.FIELDS:
         np.testing.assert_array_equal(getattr(w, name).to_numpy(), o.fields[name], err_msg=name)
+
+
+def test_driver_flow_with_taichi_stand_in_matches_oracle():
+    """The calls of a reference 3-D driver (test3d_homegenous_tensor.py:8-91:
+    ti.init, filled model fields, int receiver fields from NumPy, a moment
+    tensor set item by item with ti.sqrt, SetADEPML3D, update_SSG per step,
+    ti.sync, data.to_numpy()) made through the host stand-in for Taichi; the
+    seismograms and the final fields equal the oracle's."""
+    from seiswave_taichi_amd import taichi_shim as ti
+    from seiswave_taichi_amd.Elastic3D_SSG_Tensor import ElasticWAVE
+    ti.init(arch=ti.vulkan)
+    nx = ny = nz = 40
+    vp = ti.field(dtype=ti.f32, shape=(nx, ny, nz))
+    vp.fill(3000.)
+    vs = ti.field(dtype=ti.f32, shape=(nx, ny, nz))
+    vs.fill(3000 / 1.67)
+    rho = ti.field(dtype=ti.f32, shape=(nx, ny, nz))
+    rho.fill(2500.)
+    d, dt, order, freq, nt, nrec = 2.5, 3e-4, 3, 60.0, 70, 13
+    rs = [np.full((1, nrec), 12, dtype=int), np.full((1, nrec), 12, dtype=int),
+          np.linspace(12, 24, nrec, dtype=int).reshape(1, nrec)]
+    rsx, rsy, rsz = (ti.field(dtype=int, shape=(1, nrec)) for _ in range(3))
+    for f, a in zip((rsx, rsy, rsz), rs):
+        f.from_numpy(a)
+    MT = ti.field(dtype=ti.f32, shape=(3, 3))
+    MT[1, 2] = -1 / ti.sqrt(2)
+    MT[2, 1] = -1 / ti.sqrt(2)
+    src = (20, 20, 22)
+    w = ElasticWAVE(vs, vp, rho, d, d, d, dt, *src, rsx, rsy, rsz, MT, 1, nt, order, freq)
+    p = pml_dict(3000.0, 8, (d, d, d), freq, 3000 / 1.67)
+    w.SetADEPML3D([True] * 6, p)
+    for i in range(nt):
+        w.update_SSG(i)
+    ti.sync()
+    o = O.Oracle3D("tensor", vs.to_numpy(), vp.to_numpy(), rho.to_numpy(), d, d, d, dt, src,
+                   np.stack([r[0] for r in rs], 1), nt, order, freq=freq,
+                   mt=np.array([0, 0, 0, 0, 0, MT[1, 2]], np.float32))
+    o.set_pml([True] * 6, p)
+    o.run(nt, amps=ricker2(np.arange(nt), dt, 0.03, freq, 1.0))
+    assert np.abs(o.data).max() > 0
+    _compare(w, o)

@@ -136,16 +136,57 @@ def test_product_never_imports_the_oracle():
 def test_install_src_alias():
     import sys
     import seiswave_taichi_amd as pkg
-    saved = {k: sys.modules.pop(k) for k in list(sys.modules) if k == "src" or k.startswith("src.")}
+    def ours(k):
+        return k in ("src", "taichi") or k.startswith("src.")
+    had_taichi = "taichi" in sys.modules
+    saved = {k: sys.modules.pop(k) for k in list(sys.modules) if ours(k)}
     try:
         pkg.install_src_alias()
         from src.Elastic3D_SSG_Tensor import ElasticWAVE  # noqa: F401
+        from src.BaseKernel import Ricker  # noqa: F401  (Relay_wave.py:5)
+        from src.wigb import wigb  # noqa: F401  (test3d_homegenous_tensor.py:5)
         import src.Elastic2D_CPML as m
         assert m.__name__ == "seiswave_taichi_amd.Elastic2D_CPML"
+        import taichi as ti
+        if not had_taichi:
+            assert ti.__name__ == "seiswave_taichi_amd.taichi_shim"
     finally:
-        for k in [k for k in sys.modules if k == "src" or k.startswith("src.")]:
+        for k in [k for k in sys.modules if ours(k)]:
             sys.modules.pop(k)
         sys.modules.update(saved)
+
+
+def test_base_kernel_ricker_closed_form():
+    """src/BaseKernel.py:4-12: -s*2*a*tau*exp(-a*tau^2), tau = n*dt - 1/f; zero at
+    tau = 0, odd about it, extrema at tau = -+1/sqrt(2a) with value
+    +-s*sqrt(2a)*exp(-1/2)."""
+    from seiswave_taichi_amd.BaseKernel import Ricker, Ricker2
+    f, dt, s = 25.0, 1e-4, 1000.0
+    a = np.pi ** 2 * f ** 2
+    for n in (0, 100, 350, 400, 450, 1999):
+        tau = n * dt - 1 / f
+        assert Ricker(n, dt, f, s) == pytest.approx(-s * 2 * a * tau * np.exp(-a * tau ** 2), rel=1e-5, abs=1e-3)
+    assert abs(Ricker(400, dt, f, s)) < 1e-2 * s
+    assert Ricker(300, dt, f, s) == pytest.approx(-Ricker(500, dt, f, s), rel=1e-4)
+    n_peak = (1 / f - 1 / np.sqrt(2 * a)) / dt
+    peak = max(Ricker(n, dt, f, s) for n in range(int(n_peak) - 2, int(n_peak) + 3))
+    assert peak == pytest.approx(s * np.sqrt(2 * a) * np.exp(-0.5), rel=1e-3)
+    assert Ricker2(400, dt, 0.04, f, s) == pytest.approx(s, rel=1e-6)  # tau = 0: the peak
+
+
+def test_wigb_draws_one_wiggle_per_trace():
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    from seiswave_taichi_amd.wigb import wigb
+    t = np.linspace(0, 1, 200)[:, None]
+    a = np.sin(40 * t) * np.exp(-((t - 0.5) ** 2) / 0.01) * np.arange(1, 6)[None]
+    a[:, 2] = 0.0
+    fig = wigb(a)
+    ax = fig.axes[0]
+    assert len(ax.lines) == 5 and len(ax.collections) == 4  # the zero trace is a bare line
+    assert ax.yaxis_inverted()
+    plt.close(fig)
 
 
 def _fake_3d(n=24, order=2):
@@ -237,3 +278,23 @@ def test_header_struct_layouts_match_ctypes(tmp_path):
         assert int(out[cname]) == C.sizeof(ct), cname
         for fname, _ in ct._fields_:
             assert int(out[f"{cname}.{fname}"]) == getattr(ct, fname).offset, f"{cname}.{fname}"
+
+
+def test_taichi_shim_covers_the_driver_calls():
+    """The Taichi calls the reference drivers make outside the solver classes
+    (test3d_homegenous_tensor.py:8-58): init, fields with from_numpy and item
+    assignment, int-typed receiver fields, sqrt, sync."""
+    from seiswave_taichi_amd import taichi_shim as ti
+    ti.init(arch=ti.vulkan)
+    vp = ti.field(dtype=ti.f32, shape=(4, 5, 6))
+    vp.from_numpy(np.full((4, 5, 6), 3000.0))
+    assert vp.shape == (4, 5, 6) and vp.to_numpy().dtype == np.float32 and vp[1, 2, 3] == 3000.0
+    rsx = ti.field(dtype=int, shape=(1, 81))
+    rsx.from_numpy(np.linspace(21, 101, 81, dtype=int)[None])
+    assert rsx.to_numpy().dtype == np.int32 and rsx[0, 80] == 101
+    mt = ti.field(dtype=ti.f32, shape=(3, 3))
+    mt[1, 2] = -1 / ti.sqrt(2)
+    assert mt[1, 2] == pytest.approx(-0.70710677)
+    with pytest.raises(ValueError):
+        vp.from_numpy(np.zeros((4, 5)))
+    ti.sync()
