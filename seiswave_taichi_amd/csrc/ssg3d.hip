@@ -195,11 +195,10 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
         const float hdt = 0.5f * dt;
         // all split values of this cell first, so their loads overlap
         float qv[9];
-        const bool any_split = SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh);
 #if SW_PREFETCH && SW_SPLIT_PREFETCH
         if (SPLIT) copy_n<9>(qv, P.q);
 #else
-        if (any_split) {
+        if (SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh)) {
 #pragma unroll
             for (int t = 0; t < 9; ++t) qv[t] = qp[(long)t * NC];
         }
