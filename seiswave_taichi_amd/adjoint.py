@@ -33,6 +33,7 @@ from .coeffs import diff_coff, ricker2
 from .pml import axis_pml, check_parameter
 
 NSTATE = 15  # vx vz sxx szz sxz + 10 memory variables
+NSAVED = 5   # planes per shot and step the adjoint segment keeps (derivative sums)
 MAX_BATCH = 64  # SW_MAX_BATCH
 _FIELD_ROW = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
 
@@ -110,6 +111,11 @@ class Elastic2DModeling:
     def new_state(self, n=None):
         shape = (NSTATE, self.nz, self.nx) if n is None else (n, NSTATE, self.nz, self.nx)
         return torch.zeros(shape, dtype=torch.float32, device=self.device)
+
+    def _segment_buffer(self, n):
+        """Work of one adjoint segment call: NSAVED planes per shot and step
+        (the forward derivative sums the model gradient reads)."""
+        return torch.empty((n, NSAVED, self.nz, self.nx), dtype=torch.float32, device=self.device)
 
     def desc(self, state: torch.Tensor) -> L.SwSSG2D:
         g = L.SwSSG2D()
@@ -234,7 +240,7 @@ class Elastic2DModeling:
             q = self._adj_desc(adjb[0, :NSTATE], adjb[0, NSTATE:NSTATE + 4], adjb[0, NSTATE + 4:],
                                g_src[group[0]:] if self.force else None)
             work = self.new_state(nb)
-            seg = self.new_state(K * nb)  # 15 planes per shot and step: what the adjoint stages read
+            seg = self._segment_buffer(K * nb)
             g = self.desc(work[0])
             src = self.source(self.shots[group[0]])
             res = grad_data[group[0]:group[-1] + 1].detach().to(torch.float32).contiguous()
@@ -274,7 +280,7 @@ class Elastic2DModeling:
         acc = torch.zeros((5, nz, nx), dtype=torch.float32, device=self.device)  # g_b1 g_b4 g_lamh g_muh g_muz
         adj = self.new_state()
         tmp = torch.zeros((4, nz, nx), dtype=torch.float32, device=self.device)
-        seg = self.new_state(K)  # per step: the planes the adjoint stages read (before / after the velocity sweep)
+        seg = self._segment_buffer(K)
         work = self.new_state()
         g_src = torch.zeros(len(self.shots), dtype=torch.float32, device=self.device)
         gw = self.desc(work)

@@ -36,11 +36,16 @@ __device__ __forceinline__ float fwd_cpml(float D, float m, float a, float b, fl
 }
 
 // ------------------------------------------------------------- stress stage
-template <int N>
-__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, long fs, long as)
+// SAVED: the forward derivatives come from the planes the saved stress sweep
+// wrote (sv + blockIdx.z * ss, planes 2-4) instead of being recomputed from
+// the forward state; they are the same values, so the result is the same.
+template <int N, bool SAVED>
+__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, long fs, long as, const float *sv,
+                                                      long ss)
 {
     shift_state(f, (long)blockIdx.z * fs);
     shift_adj(q, (long)blockIdx.z * as);
+    if (SAVED) sv += (long)blockIdx.z * ss;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
@@ -57,13 +62,18 @@ __global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, lo
         const float lam_h = 0.5f * (f.lam[p] + f.lam[p + 1]);
         const float mu_h = 0.5f * (f.mu[p] + f.mu[p + 1]);
         const float lpm = lam_h + 2.f * mu_h;
-        float Dx = d_fm<N>(f.vx, p, 1, c) / dx;
-        float Dz = d_bm<N>(f.vz, p, SZ, c) / dz;
         const bool in = X.in_h[i] | Z.in[j];
-        float Dxp = Dx, Dzp = Dz;
-        if (in) {
-            Dxp = fwd_cpml(Dx, f.mem[6 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
-            Dzp = fwd_cpml(Dz, f.mem[9 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
+        float Dxp, Dzp;
+        if (SAVED) {
+            Dxp = sv[2 * NC + p];
+            Dzp = sv[3 * NC + p];
+        } else {
+            Dxp = d_fm<N>(f.vx, p, 1, c) / dx;
+            Dzp = d_bm<N>(f.vz, p, SZ, c) / dz;
+            if (in) {
+                Dxp = fwd_cpml(Dxp, f.mem[6 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
+                Dzp = fwd_cpml(Dzp, f.mem[9 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
+            }
         }
         // d/d lam_h = (gxx+gzz)(Dx'+Dz'); d/d mu_h = 2(gxx Dx' + gzz Dz').  The vs and
         // rho gradients need mu_bar - 2 lam_bar, whose two terms nearly cancel for
@@ -83,15 +93,20 @@ __global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, lo
     if (i >= N + 1 && i < nx - N && j >= N && j < nz - N - 1) {
         const float g = q.sxz[p] * dt;
         const float mu_z = 0.5f * (f.mu[p + SZ] + f.mu[p]);
-        float Dx = d_bm<N>(f.vz, p, 1, c) / dx;
-        float Dz = d_fm<N>(f.vx, p, SZ, c) / dz;
         const bool in = X.in[i] | Z.in_h[j];
-        float Dxp = Dx, Dzp = Dz;
-        if (in) {
-            Dxp = fwd_cpml(Dx, f.mem[8 * NC + p], X.a[i], X.b[i], X.k[i]);
-            Dzp = fwd_cpml(Dz, f.mem[7 * NC + p], Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+        float sum;
+        if (SAVED) {
+            sum = sv[4 * NC + p];
+        } else {
+            float Dxp = d_bm<N>(f.vz, p, 1, c) / dx;
+            float Dzp = d_fm<N>(f.vx, p, SZ, c) / dz;
+            if (in) {
+                Dxp = fwd_cpml(Dxp, f.mem[8 * NC + p], X.a[i], X.b[i], X.k[i]);
+                Dzp = fwd_cpml(Dzp, f.mem[7 * NC + p], Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+            }
+            sum = Dzp + Dxp;
         }
-        q.g_muz[p] += g * (Dzp + Dxp);
+        q.g_muz[p] += g * sum;
         float bx = mu_z * g, bz = mu_z * g;
         if (in) {
             bx = cpml_adjoint(bx, q.mem + 8 * NC + p, X.a[i], X.b[i], X.k[i]);
@@ -141,11 +156,14 @@ __global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q, lo
 }
 
 // ----------------------------------------------------------- velocity stage
-template <int N>
-__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, long fs, long as)
+// SAVED: planes 0-1 of the saved velocity sweep, as in k_adj_stress_a.
+template <int N, bool SAVED>
+__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, long fs, long as, const float *sv,
+                                                        long ss)
 {
     shift_state(f, (long)blockIdx.z * fs);
     shift_adj(q, (long)blockIdx.z * as);
+    if (SAVED) sv += (long)blockIdx.z * ss;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
@@ -160,15 +178,20 @@ __global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, 
     if (i >= N + 1 && i < nx - N && j >= N + 1 && j < nz - N) {
         const float vb = q.vx[p];
         const float g = vb * dt / f.rho[p];
-        float Dx = d_bm<N>(f.sxx, p, 1, c) / dx;
-        float Dz = d_bm<N>(f.sxz, p, SZ, c) / dz;
         const bool in = X.in[i] | Z.in[j];
-        float Dxp = Dx, Dzp = Dz;
-        if (in) {
-            Dxp = fwd_cpml(Dx, f.mem[0 * NC + p], X.a[i], X.b[i], X.k[i]);
-            Dzp = fwd_cpml(Dz, f.mem[3 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
+        float sum;
+        if (SAVED) {
+            sum = sv[p];
+        } else {
+            float Dxp = d_bm<N>(f.sxx, p, 1, c) / dx;
+            float Dzp = d_bm<N>(f.sxz, p, SZ, c) / dz;
+            if (in) {
+                Dxp = fwd_cpml(Dxp, f.mem[0 * NC + p], X.a[i], X.b[i], X.k[i]);
+                Dzp = fwd_cpml(Dzp, f.mem[3 * NC + p], Z.a[j], Z.b[j], Z.k[j]);
+            }
+            sum = Dxp + Dzp;
         }
-        q.g_b1[p] += vb * (Dxp + Dzp);
+        q.g_b1[p] += vb * sum;
         float bx = g, bz = g;
         if (in) {
             bx = cpml_adjoint(bx, q.mem + 0 * NC + p, X.a[i], X.b[i], X.k[i]);
@@ -182,15 +205,20 @@ __global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, 
         const float vb = q.vz[p];
         const float rho4 = 0.25f * (f.rho[p] + f.rho[p + SZ] + f.rho[p + 1 + SZ] + f.rho[p + 1]);
         const float g = vb * dt / rho4;
-        float Dx = d_fm<N>(f.sxz, p, 1, c) / dx;
-        float Dz = d_fm<N>(f.szz, p, SZ, c) / dz;
         const bool in = X.in_h[i] | Z.in_h[j];
-        float Dxp = Dx, Dzp = Dz;
-        if (in) {
-            Dxp = fwd_cpml(Dx, f.mem[2 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
-            Dzp = fwd_cpml(Dz, f.mem[5 * NC + p], Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+        float sum;
+        if (SAVED) {
+            sum = sv[NC + p];
+        } else {
+            float Dxp = d_fm<N>(f.sxz, p, 1, c) / dx;
+            float Dzp = d_fm<N>(f.szz, p, SZ, c) / dz;
+            if (in) {
+                Dxp = fwd_cpml(Dxp, f.mem[2 * NC + p], X.a_h[i], X.b_h[i], X.k_h[i]);
+                Dzp = fwd_cpml(Dzp, f.mem[5 * NC + p], Z.a_h[j], Z.b_h[j], Z.k_h[j]);
+            }
+            sum = Dzp + Dxp;
         }
-        q.g_b4[p] += vb * (Dzp + Dxp);
+        q.g_b4[p] += vb * sum;
         float bx = g, bz = g;
         if (in) {
             bx = cpml_adjoint(bx, q.mem + 2 * NC + p, X.a_h[i], X.b_h[i], X.k_h[i]);
@@ -308,35 +336,46 @@ inline dim3 grid2(const sw_ssg2d &g, dim3 blk, int nbatch = 1)
 
 }  // namespace
 
-int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, hipStream_t st)
+// sv == nullptr: recompute the forward derivatives from the state f holds
+// (shot b's at + b * fs); otherwise read them from the saved planes.
+int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, const float *sv,
+                              long ss, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(f->order, {
-        hipLaunchKernelGGL((k_adj_stress_a<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
-        hipLaunchKernelGGL((k_adj_stress_b<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
+        const dim3 grd = grid2(*f, blk, nbatch);
+        if (sv)
+            hipLaunchKernelGGL((k_adj_stress_a<N, true>), grd, blk, 0, st, *f, *q, fs, as, sv, ss);
+        else
+            hipLaunchKernelGGL((k_adj_stress_a<N, false>), grd, blk, 0, st, *f, *q, fs, as, nullptr, 0L);
+        hipLaunchKernelGGL((k_adj_stress_b<N>), grd, blk, 0, st, *f, *q, fs, as);
     });
     return launch_status("2-D adjoint stress stage");
 }
 
 int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as,
-                                hipStream_t st)
+                                const float *sv, long ss, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(f->order, {
-        hipLaunchKernelGGL((k_adj_velocity_a<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
-        hipLaunchKernelGGL((k_adj_velocity_b<N>), grid2(*f, blk, nbatch), blk, 0, st, *f, *q, fs, as);
+        const dim3 grd = grid2(*f, blk, nbatch);
+        if (sv)
+            hipLaunchKernelGGL((k_adj_velocity_a<N, true>), grd, blk, 0, st, *f, *q, fs, as, sv, ss);
+        else
+            hipLaunchKernelGGL((k_adj_velocity_a<N, false>), grd, blk, 0, st, *f, *q, fs, as, nullptr, 0L);
+        hipLaunchKernelGGL((k_adj_velocity_b<N>), grd, blk, 0, st, *f, *q, fs, as);
     });
     return launch_status("2-D adjoint velocity stage");
 }
 
 int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
 {
-    return launch_adj2d_stress_batch(f, q, 1, 0, 0, st);
+    return launch_adj2d_stress_batch(f, q, 1, 0, 0, nullptr, 0, st);
 }
 
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)
 {
-    return launch_adj2d_velocity_batch(f, q, 1, 0, 0, st);
+    return launch_adj2d_velocity_batch(f, q, 1, 0, 0, nullptr, 0, st);
 }
 
 int launch_adj2d_inject_batch(const sw_adj2d *q, const sw_receivers *r, int it, int nbatch, long as, long ds,

@@ -37,9 +37,12 @@ int mlp_param_count();
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
 int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st);
 int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st);
-int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, hipStream_t st);
+int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, const float *sv,
+                              long ss, hipStream_t st);
 int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as,
-                                hipStream_t st);
+                                const float *sv, long ss, hipStream_t st);
+int launch_ssg2d_sweeps_saved(const sw_ssg2d *g, int nbatch, long bs, bool stress, float *sv, long ss,
+                              hipStream_t st);
 int launch_adj2d_inject_batch(const sw_adj2d *q, const sw_receivers *r, int it, int nbatch, long as, long ds,
                               hipStream_t st);
 int launch_adj2d_model_gradient_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, const float *vp,
@@ -655,53 +658,32 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
     }
     if ((rc = check_batch_receivers(g, residual, it0, n))) return rc;
     SW_REQUIRE(!force_source(src) || adj->g_src, "force sources need the adj->g_src accumulator");
-    const long P = (long)g->nx * g->nz, S = 15 * P;
+    const long P = (long)g->nx * g->nz;
     SW_REQUIRE(nbatch == 1 || adj_stride >= 24 * P, "adjoint stride %ld below one adjoint block (%ld floats)",
                (long)adj_stride, 24 * P);
     const long bs = nbatch == 1 ? 0 : (long)state_stride, as = nbatch == 1 ? 0 : (long)adj_stride;
-    // Kept per step, per shot: what the two adjoint stages read of the forward
-    // state.  The velocity stage reads sxx szz sxz and memory variables 0..5
-    // (state planes 2..10, before the velocity sweep); the stress stage reads
-    // vx vz and memory variables 6..9 (planes 0..1 and 11..14, after it).
-    const long pre_n = 9 * P, mid_n = 6 * P;
-    const long step_n = (long)nbatch * (pre_n + mid_n);
-    const size_t pitch = sizeof(float) * (size_t)(nbatch == 1 ? S : (long)state_stride);
+    // Kept per step and shot: the 5 planes of CPML-corrected derivative sums
+    // the saved sweeps write, all the model gradient needs of the forward
+    // state; the adjoint stages themselves are linear in the adjoint state.
+    const long save_n = 5 * P, step_n = (long)nbatch * save_n;
     hipStream_t st = as_stream(stream);
-    auto rows = [&](float *dst, long dst_pitch, const float *src0, long width) -> hipError_t {
-        if (nbatch == 1)
-            return hipMemcpyAsync(dst, src0, sizeof(float) * width, hipMemcpyDeviceToDevice, st);
-        return hipMemcpy2DAsync(dst, sizeof(float) * dst_pitch, src0, pitch, sizeof(float) * width, nbatch,
-                                hipMemcpyDeviceToDevice, st);
-    };
     for (int t = 0; t < n; ++t) {
-        float *pre = work + (size_t)t * step_n, *mid = pre + (size_t)nbatch * pre_n;
+        float *sv = work + (size_t)t * step_n;
         if (stress_source(src))
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
-        SW_TRY_HIP(rows(pre, pre_n, g->vx + 2 * P, pre_n));
-        if ((rc = launch_ssg2d_velocity_batch(g, nbatch, bs, st))) return rc;
+        if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, false, sv, save_n, st))) return rc;
         if (force_source(src))
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
-        SW_TRY_HIP(rows(mid, mid_n, g->vx, 2 * P));
-        SW_TRY_HIP(rows(mid + 2 * P, mid_n, g->vx + 11 * P, 4 * P));
-        if ((rc = launch_ssg2d_stress_batch(g, nbatch, bs, st))) return rc;
+        if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, true, sv, save_n, st))) return rc;
     }
     for (int t = n - 1; t >= 0; --t) {
-        float *pre = work + (size_t)t * step_n, *mid = pre + (size_t)nbatch * pre_n;
+        const float *sv = work + (size_t)t * step_n;
         if (residual->nrec && (rc = launch_adj2d_inject_batch(adj, residual, it0 + t, nbatch, as, data_stride, st)))
             return rc;
-        // fields a stage does not read point at valid memory of the same step
-        sw_ssg2d fm = *g;
-        fm.vx = mid; fm.vz = mid + P;
-        fm.mem = mid - 4 * P;  // memory variables 6..9 at mid + 2P .. mid + 6P
-        fm.sxx = fm.szz = fm.sxz = mid;
-        if ((rc = launch_adj2d_stress_batch(&fm, adj, nbatch, mid_n, as, st))) return rc;
+        if ((rc = launch_adj2d_stress_batch(g, adj, nbatch, 0, as, sv, save_n, st))) return rc;
         if (force_source(src))
             hipLaunchKernelGGL(k_adj_force_batch, dim3(nbatch), dim3(1), 0, st, *adj, *src, sc, it0 + t, as, g->nx);
-        sw_ssg2d fp = *g;
-        fp.sxx = pre; fp.szz = pre + P; fp.sxz = pre + 2 * P;
-        fp.mem = pre + 3 * P;  // memory variables 0..5
-        fp.vx = fp.vz = pre;
-        if ((rc = launch_adj2d_velocity_batch(&fp, adj, nbatch, pre_n, as, st))) return rc;
+        if ((rc = launch_adj2d_velocity_batch(g, adj, nbatch, 0, as, sv, save_n, st))) return rc;
     }
     return launch_status("ssg2d adjoint segment batch");
 }

@@ -49,10 +49,17 @@ struct Rot {
     }
 };
 
-template <int N, int SCHEME>
-__global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs)
+// With SAVE (VS scheme), each sweep also writes the CPML-corrected derivative
+// sums its updates multiply, which are all the model gradient needs of the
+// forward state: planes 0-1 of sv (velocity sweep: vx cells dsxx/dx +
+// dsxz/dz, vz cells dszz/dz + dsxz/dx) and 2-4 (stress sweep: sxx cells
+// dvx/dx and dvz/dz, sxz cells dvx/dz + dvz/dx), at the cells each sweep
+// updates; shot b's planes start b*ss floats after sv.
+template <int N, int SCHEME, bool SAVE>
+__global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs, float *sv, long ss)
 {
     shift_state(g, (long)blockIdx.z * bs);
+    if (SAVE) sv += (long)blockIdx.z * ss;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
@@ -73,7 +80,9 @@ __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs)
                 dsxxdx = cpml_apply(dsxxdx, m + 0 * NC + p, X.a[i], X.b[i], X.k[i]);
                 dsxzdz = cpml_apply(dsxzdz, m + 3 * NC + p, Z.a[j], Z.b[j], Z.k[j]);
             }
-            g.vx[p] += (dsxxdx + dsxzdz) * dt / g.rho[p];
+            const float sum = dsxxdx + dsxzdz;
+            g.vx[p] += sum * dt / g.rho[p];
+            if (SAVE) sv[p] = sum;
         }
         // vz at (i+1/2, j+1/2): :156-167
         if (i < nx - N - 1 && j < nz - N - 1) {
@@ -84,7 +93,9 @@ __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs)
                 dsxzdx = cpml_apply(dsxzdx, m + 2 * NC + p, X.a_h[i], X.b_h[i], X.k_h[i]);
                 dszzdz = cpml_apply(dszzdz, m + 5 * NC + p, Z.a_h[j], Z.b_h[j], Z.k_h[j]);
             }
-            g.vz[p] += (dszzdz + dsxzdx) * dt / rho_avg;
+            const float sum = dszzdz + dsxzdx;
+            g.vz[p] += sum * dt / rho_avg;
+            if (SAVE) sv[NC + p] = sum;
         }
     } else if (SCHEME == SW_2D_SV) {
         // vx at (i+1/2, j): :218-229
@@ -133,10 +144,11 @@ __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs)
     }
 }
 
-template <int N, int SCHEME>
-__global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs)
+template <int N, int SCHEME, bool SAVE>
+__global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs, float *sv, long ss)
 {
     shift_state(g, (long)blockIdx.z * bs);
+    if (SAVE) sv += (long)blockIdx.z * ss;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
@@ -162,6 +174,10 @@ __global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs)
             }
             g.sxx[p] += (lpm * dvxdx + lam_h * dvzdz) * dt;
             g.szz[p] += (lpm * dvzdz + lam_h * dvxdx) * dt;
+            if (SAVE) {
+                sv[2 * NC + p] = dvxdx;
+                sv[3 * NC + p] = dvzdz;
+            }
         }
         // sxz at (i, j+1/2): :188-199
         if (i >= N + 1 && j < nz - N - 1) {
@@ -172,7 +188,9 @@ __global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs)
                 dvzdx = cpml_apply(dvzdx, m + 8 * NC + p, X.a[i], X.b[i], X.k[i]);
                 dvxdz = cpml_apply(dvxdz, m + 7 * NC + p, Z.a_h[j], Z.b_h[j], Z.k_h[j]);
             }
-            g.sxz[p] += mu_half_z * (dvxdz + dvzdx) * dt;
+            const float sum = dvxdz + dvzdx;
+            g.sxz[p] += mu_half_z * sum * dt;
+            if (SAVE) sv[4 * NC + p] = sum;
         }
     } else if (SCHEME == SW_2D_SV) {
         // sxx, szz at (i, j): :247-260
@@ -236,9 +254,9 @@ int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, hipStrea
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
-        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS>), grd, blk, 0, st, *g, bs);
-        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_SV>), grd, blk, 0, st, *g, bs);
-        else hipLaunchKernelGGL((k_velocity2d<N, SW_2D_RSG>), grd, blk, 0, st, *g, bs);
+        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
+        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_SV, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
+        else hipLaunchKernelGGL((k_velocity2d<N, SW_2D_RSG, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
     });
     return launch_status("ssg2d velocity");
 }
@@ -248,11 +266,26 @@ int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
-        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS>), grd, blk, 0, st, *g, bs);
-        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_stress2d<N, SW_2D_SV>), grd, blk, 0, st, *g, bs);
-        else hipLaunchKernelGGL((k_stress2d<N, SW_2D_RSG>), grd, blk, 0, st, *g, bs);
+        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
+        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_stress2d<N, SW_2D_SV, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
+        else hipLaunchKernelGGL((k_stress2d<N, SW_2D_RSG, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
     });
     return launch_status("ssg2d stress");
+}
+
+int launch_ssg2d_sweeps_saved(const sw_ssg2d *g, int nbatch, long bs, bool stress, float *sv, long ss,
+                              hipStream_t st)
+{
+    const dim3 blk(64, 4, 1);
+    SW_REQUIRE(g->scheme == SW_2D_VS, "saved sweeps exist for the VS scheme only");
+    SW_DISPATCH_ORDER(g->order, {
+        const dim3 grd = box_grid2(*g, N, blk, nbatch);
+        if (stress)
+            hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS, true>), grd, blk, 0, st, *g, bs, sv, ss);
+        else
+            hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS, true>), grd, blk, 0, st, *g, bs, sv, ss);
+    });
+    return launch_status(stress ? "ssg2d stress (saved)" : "ssg2d velocity (saved)");
 }
 
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st)

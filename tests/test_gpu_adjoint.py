@@ -112,6 +112,72 @@ def test_chain_transpose_dot_product():
     assert abs(lhs - rhs) <= 1e-4 * max(abs(lhs), abs(rhs), 1.0), (lhs, rhs)
 
 
+@pytest.mark.parametrize("source", ["explosive", "force_vz"])
+def test_segment_matches_stage_by_stage_schedule(source):
+    """sw_ssg2d_adjoint_segment (forward derivative sums saved by the
+    recomputed sweeps) leaves the adjoint state and the five coefficient
+    accumulators bit-equal to the per-stage calls on full forward states kept
+    before and after every velocity sweep (which recompute those sums)."""
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd.adjoint import Elastic2DModeling
+    d, nt, K = (10.0, 10.0), 60, 25
+    shape, shot = (56, 44), (27, 15)
+    rx = np.arange(8, shape[0] - 8, 5)
+    rec = np.stack([rx, np.full_like(rx, 5)], 1)
+    p = pml_dict(4700.0, 8, d, 20.0, 2700.0, dims=2)
+    wavelet = 1e8 * ricker2(np.arange(nt), 8e-4, 1.5 / 20.0, 20.0, 1.0)
+    m = Elastic2DModeling(shape, *d, 8e-4, nt, 2, p, [shot], rec, freq=20.0, checkpoint=K, wavelet=wavelet,
+                          source=source, batch=False)
+    vs, vp, rho = layered_model(shape, seed=6)
+    m.set_model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
+    lib, st = m.lib, m.stream()
+    force = source != "explosive"
+    src = m.source(shot)
+    data = torch.zeros((nt, len(m.components), m.nrec), device=m.device)
+    rec_d = m.receivers(data)
+    state = m.new_state()
+    g = m.desc(state)
+    pre, mid, ck = [], [], []
+    for it in range(nt):
+        if it % K == 0:
+            ck.append(state.clone())
+        if not force:
+            L.check(lib.sw_ssg2d_inject(g, src, it, st), "inject")
+        pre.append(state.clone())
+        L.check(lib.sw_ssg2d_velocity(g, st), "velocity")
+        if force:
+            L.check(lib.sw_ssg2d_inject(g, src, it, st), "inject")
+        mid.append(state.clone())
+        L.check(lib.sw_ssg2d_stress(g, st), "stress")
+        L.check(lib.sw_ssg2d_record(g, rec_d, it, st), "record")
+    gen = torch.Generator(device=m.device).manual_seed(9)
+    res = torch.randn(data.shape, generator=gen, device=m.device) * data.abs().max()
+    rres = m.receivers(res)
+    out = []
+    for staged in (True, False):
+        adj = m.new_state()
+        tmp = torch.zeros((4, m.nz, m.nx), device=m.device)
+        acc = torch.zeros((5, m.nz, m.nx), device=m.device)
+        g_src = torch.zeros(1, device=m.device)
+        q = m._adj_desc(adj, tmp, acc, g_src if force else None)
+        if staged:
+            for it in reversed(range(nt)):
+                L.check(lib.sw_ssg2d_adjoint_inject(g, q, rres, it, st), "adjoint inject")
+                L.check(lib.sw_ssg2d_adjoint_stress(m.desc(mid[it]), q, st), "adjoint stress")
+                L.check(lib.sw_ssg2d_adjoint_velocity(m.desc(pre[it]), q, st), "adjoint velocity")
+        else:
+            work = m.new_state()
+            seg = m._segment_buffer(K)
+            for s0 in reversed(range(0, nt, K)):
+                work.copy_(ck[s0 // K])
+                L.check(lib.sw_ssg2d_adjoint_segment(m.desc(work), src, q, rres, L.fptr(seg), s0, min(K, nt - s0),
+                                                     st), "adjoint segment")
+        torch.cuda.synchronize()
+        out.append((adj.clone(), acc.clone()))
+    assert out[0][1].abs().max() > 0
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
 def test_forward_modeling_matches_oracle():
     m, p, rec = _modeling()
     vs, vp, rho = layered_model((m.nx, m.nz), seed=3)
