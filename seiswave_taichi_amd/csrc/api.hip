@@ -35,14 +35,14 @@ int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, f
                 hipStream_t st);
 int mlp_param_count();
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
-int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st);
-int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st);
+int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st);
+int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st);
 int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, const float *sv,
                               long ss, hipStream_t st);
 int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as,
                                 const float *sv, long ss, hipStream_t st);
 int launch_ssg2d_sweeps_saved(const sw_ssg2d *g, int nbatch, long bs, bool stress, float *sv, long ss,
-                              hipStream_t st);
+                              const Fold2D &fo, hipStream_t st);
 int launch_adj2d_inject_batch(const sw_adj2d *q, const sw_receivers *r, int it, int nbatch, long as, long ds,
                               hipStream_t st);
 int launch_adj2d_model_gradient_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, const float *vp,
@@ -147,18 +147,10 @@ __device__ __forceinline__ void inject2d(sw_ssg2d &g, const sw_source &s, int it
         return;
     }
     switch (s.kind) {
-    case SW_SRC_FORCE_VZ: g.vz[p] += g.dt * amp / g.rho[p]; break;
-    case SW_SRC_FORCE_VX: g.vx[p] += g.dt * amp / g.rho[p]; break;
-    case SW_SRC_MOMENT:  // 2-D moment tensor (Mxx, Mzz, Mxz taken from mt[0], mt[2], mt[4])
-        g.sxx[p] += amp * (-s.mt[0]);
-        g.szz[p] += amp * (-s.mt[2]);
-        g.sxz[p] += amp * (-s.mt[4]);
-        break;
-    case SW_SRC_EXPLOSIVE: {
-        const float e = amp * (-0.70710678118654752f);
-        g.sxx[p] += e;
-        g.szz[p] += e;
-    } break;
+    case SW_SRC_FORCE_VZ:
+    case SW_SRC_FORCE_VX: force_src2d(g, s.kind, amp, p); break;
+    case SW_SRC_MOMENT:
+    case SW_SRC_EXPLOSIVE: stress_src2d(g, s.kind, s.mt[0], s.mt[2], s.mt[4], amp, p); break;
     default: break;
     }
 }
@@ -196,6 +188,51 @@ __global__ void k_adj_force_batch(sw_adj2d q, sw_source s, ShotCells sc, int it,
 
 inline bool force_source(const sw_source *s) { return s->kind == SW_SRC_FORCE_VZ || s->kind == SW_SRC_FORCE_VX; }
 inline bool stress_source(const sw_source *s) { return s->kind == SW_SRC_MOMENT || s->kind == SW_SRC_EXPLOSIVE; }
+// What of a 2-D step's source injection and receiver gather the sweeps can
+// do themselves (Fold2D): not on the rotated grid (its force spreads over
+// four cells), only for source cells that own a sweep thread ([N, n-N) on
+// both axes), receivers recording velocities only, and a next-step stress
+// source only when the gather is folded too (a separate gather after the
+// stress sweep would see it).  Unset parts run as separate launches.
+Fold2D plan_fold2d(const sw_ssg2d *g, const sw_source *src, const sw_receivers *rec, int nbatch, const int *ix,
+                   const int *iz, long data_stride)
+{
+    Fold2D fo{};
+    const int N = g->order;
+    bool cells_ok = g->scheme != SW_2D_RSG && src->kind != SW_SRC_NONE;
+    for (int b = 0; b < nbatch && cells_ok; ++b) {
+        cells_ok = ix[b] >= N && ix[b] < g->nx - N && iz[b] >= N && iz[b] < g->nz - N;
+        fo.ix[b] = ix[b];
+        fo.iz[b] = iz[b];
+    }
+    bool rec_ok = rec && rec->nrec > 0;
+    for (int c = 0; rec_ok && c < rec->ncomp; ++c) rec_ok = rec->comp[c] == 0 || rec->comp[c] == 2;
+    if (rec_ok) {
+        fo.record = 1;
+        fo.rec = *rec;
+        fo.ds = data_stride;
+    }
+    const bool no_gather_after = !rec || rec->nrec == 0 || rec_ok;
+    if (cells_ok && force_source(src)) fo.force = src->kind;
+    if (cells_ok && stress_source(src) && no_gather_after) fo.stress = src->kind;
+    fo.amp = src->amp;
+    fo.m0 = src->mt[0];
+    fo.m2 = src->mt[2];
+    fo.m4 = src->mt[4];
+    return fo;
+}
+
+// The stress-source fold injects step it+1's source in step it's stress
+// sweep: the first step of a call injects its own, the last one nothing.
+inline void fold_step(Fold2D &fo, const Fold2D &plan, int it, int it_end)
+{
+    fo.force = plan.force;
+    fo.it_force = it;
+    fo.stress = it + 1 < it_end ? plan.stress : 0;
+    fo.it_next = it + 1;
+    fo.record = plan.record;
+    fo.it_rec = it;
+}
 
 // Receivers of shot blockIdx.y: state at + y * bs, traces at + y * ds.
 __global__ void k_record2d_batch(sw_ssg2d g, sw_receivers r, int it, long bs, long ds)
@@ -470,14 +507,18 @@ int sw_ssg2d_run(const sw_ssg2d *g, const sw_source *src, const sw_receivers *re
             SW_REQUIRE(fields2d(g).f[rec->comp[c]] != nullptr, "component %d does not exist in 2-D", rec->comp[c]);
     }
     hipStream_t st = as_stream(stream);
-    const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
     const FieldPtrs fp = fields2d(g);
-    for (int it = it0; it < it0 + nsteps; ++it) {
-        if (pre) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it);
-        if ((rc = launch_ssg2d_velocity(g, st))) return rc;
-        if (!pre && src->kind != SW_SRC_NONE) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it);
-        if ((rc = launch_ssg2d_stress(g, st))) return rc;
-        if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
+    const int ix = src->ix, iz = src->iz, it_end = it0 + nsteps;
+    const Fold2D plan = plan_fold2d(g, src, rec, 1, &ix, &iz, 0);
+    Fold2D fo = plan;
+    for (int it = it0; it < it_end; ++it) {
+        fold_step(fo, plan, it, it_end);
+        if (stress_source(src) && (it == it0 || !plan.stress))
+            hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg2d_velocity_batch(g, 1, 0, fo, st))) return rc;
+        if (force_source(src) && !plan.force) hipLaunchKernelGGL(k_inject2d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg2d_stress_batch(g, 1, 0, fo, st))) return rc;
+        if (rec && !plan.record && (rc = record_launch(fp, rec, it, st))) return rc;
     }
     return launch_status("ssg2d run");
 }
@@ -625,17 +666,22 @@ int sw_ssg2d_run_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, 
     }
     if (rec && (rc = check_batch_receivers(g, rec, it0, nsteps))) return rc;
     hipStream_t st = as_stream(stream);
-    for (int it = it0; it < it0 + nsteps; ++it) {
-        if (stress_source(src))
-            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, (long)state_stride);
-        if ((rc = launch_ssg2d_velocity_batch(g, nbatch, state_stride, st))) return rc;
-        if (force_source(src))
-            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, (long)state_stride);
-        if ((rc = launch_ssg2d_stress_batch(g, nbatch, state_stride, st))) return rc;
-        if (rec && rec->nrec) {
+    const int it_end = it0 + nsteps;
+    const long bs = nbatch == 1 ? 0 : (long)state_stride, ds = nbatch == 1 ? 0 : (long)data_stride;
+    const Fold2D plan = plan_fold2d(g, src, rec, nbatch, sc.ix, sc.iz, ds);
+    Fold2D fo = plan;
+    for (int it = it0; it < it_end; ++it) {
+        fold_step(fo, plan, it, it_end);
+        if (stress_source(src) && (it == it0 || !plan.stress))
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, bs);
+        if ((rc = launch_ssg2d_velocity_batch(g, nbatch, bs, fo, st))) return rc;
+        if (force_source(src) && !plan.force)
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, bs);
+        if ((rc = launch_ssg2d_stress_batch(g, nbatch, bs, fo, st))) return rc;
+        if (rec && rec->nrec && !plan.record) {
             const int n = rec->nrec * rec->ncomp;
-            hipLaunchKernelGGL(k_record2d_batch, dim3((n + 255) / 256, nbatch), dim3(256), 0, st, *g, *rec, it,
-                               (long)state_stride, (long)data_stride);
+            hipLaunchKernelGGL(k_record2d_batch, dim3((n + 255) / 256, nbatch), dim3(256), 0, st, *g, *rec, it, bs,
+                               ds);
         }
     }
     return launch_status("ssg2d batch run");
@@ -667,14 +713,18 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
     // state; the adjoint stages themselves are linear in the adjoint state.
     const long save_n = 5 * P, step_n = (long)nbatch * save_n;
     hipStream_t st = as_stream(stream);
+    const Fold2D plan = plan_fold2d(g, src, nullptr, nbatch, sc.ix, sc.iz, 0);
+    Fold2D fo = plan;
     for (int t = 0; t < n; ++t) {
         float *sv = work + (size_t)t * step_n;
-        if (stress_source(src))
-            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
-        if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, false, sv, save_n, st))) return rc;
-        if (force_source(src))
-            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it0 + t, bs);
-        if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, true, sv, save_n, st))) return rc;
+        const int it = it0 + t;
+        fold_step(fo, plan, it, it0 + n);
+        if (stress_source(src) && (t == 0 || !plan.stress))
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, bs);
+        if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, false, sv, save_n, fo, st))) return rc;
+        if (force_source(src) && !plan.force)
+            hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, bs);
+        if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, true, sv, save_n, fo, st))) return rc;
     }
     for (int t = n - 1; t >= 0; --t) {
         const float *sv = work + (size_t)t * step_n;

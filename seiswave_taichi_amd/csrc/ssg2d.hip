@@ -56,7 +56,7 @@ struct Rot {
 // dvx/dx and dvz/dz, sxz cells dvx/dz + dvz/dx), at the cells each sweep
 // updates; shot b's planes start b*ss floats after sv.
 template <int N, int SCHEME, bool SAVE>
-__global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs, float *sv, long ss)
+__global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs, float *sv, long ss, Fold2D fo)
 {
     shift_state(g, (long)blockIdx.z * bs);
     if (SAVE) sv += (long)blockIdx.z * ss;
@@ -142,13 +142,24 @@ __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs, float *
             g.vz[p] += (dszzdz + dsxzdx) * dt / rho;
         }
     }
+    if (SCHEME != SW_2D_RSG && fo.force && i == fo.ix[blockIdx.z] && j == fo.iz[blockIdx.z])
+        force_src2d(g, fo.force, fo.amp[fo.it_force], p);
 }
 
 template <int N, int SCHEME, bool SAVE>
-__global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs, float *sv, long ss)
+__global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs, float *sv, long ss, Fold2D fo)
 {
     shift_state(g, (long)blockIdx.z * bs);
     if (SAVE) sv += (long)blockIdx.z * ss;
+    if (fo.record && blockIdx.x == 0 && blockIdx.y == 0) {
+        // velocity receivers of this step: data[(it*ncomp + comp)*nrec + r]
+        const int n = fo.rec.nrec * fo.rec.ncomp;
+        float *out = fo.rec.data + (long)blockIdx.z * fo.ds + (long)fo.it_rec * n;
+        for (int t = threadIdx.y * blockDim.x + threadIdx.x; t < n; t += blockDim.x * blockDim.y) {
+            const int comp = t / fo.rec.nrec, r = t - comp * fo.rec.nrec;
+            out[t] = (fo.rec.comp[comp] == 0 ? g.vx : g.vz)[fo.rec.idx[r]];
+        }
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
@@ -240,6 +251,8 @@ __global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs, float *sv
             g.sxz[p] += (mu * (dvxdz + dvzdx)) * dt;
         }
     }
+    if (SCHEME != SW_2D_RSG && fo.stress && i == fo.ix[blockIdx.z] && j == fo.iz[blockIdx.z])
+        stress_src2d(g, fo.stress, fo.m0, fo.m2, fo.m4, fo.amp[fo.it_next], p);
 }
 
 inline dim3 box_grid2(const sw_ssg2d &g, int N, dim3 blk, int nbatch)
@@ -249,53 +262,59 @@ inline dim3 box_grid2(const sw_ssg2d &g, int N, dim3 blk, int nbatch)
 
 }  // namespace
 
-int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st)
+int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
-        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
-        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_velocity2d<N, SW_2D_SV, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
-        else hipLaunchKernelGGL((k_velocity2d<N, SW_2D_RSG, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
+        if (g->scheme == SW_2D_VS)
+            hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS, false>), grd, blk, 0, st, *g, bs, nullptr, 0L, fo);
+        else if (g->scheme == SW_2D_SV)
+            hipLaunchKernelGGL((k_velocity2d<N, SW_2D_SV, false>), grd, blk, 0, st, *g, bs, nullptr, 0L, fo);
+        else
+            hipLaunchKernelGGL((k_velocity2d<N, SW_2D_RSG, false>), grd, blk, 0, st, *g, bs, nullptr, 0L, fo);
     });
     return launch_status("ssg2d velocity");
 }
 
-int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, hipStream_t st)
+int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
-        if (g->scheme == SW_2D_VS) hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
-        else if (g->scheme == SW_2D_SV) hipLaunchKernelGGL((k_stress2d<N, SW_2D_SV, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
-        else hipLaunchKernelGGL((k_stress2d<N, SW_2D_RSG, false>), grd, blk, 0, st, *g, bs, nullptr, 0L);
+        if (g->scheme == SW_2D_VS)
+            hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS, false>), grd, blk, 0, st, *g, bs, nullptr, 0L, fo);
+        else if (g->scheme == SW_2D_SV)
+            hipLaunchKernelGGL((k_stress2d<N, SW_2D_SV, false>), grd, blk, 0, st, *g, bs, nullptr, 0L, fo);
+        else
+            hipLaunchKernelGGL((k_stress2d<N, SW_2D_RSG, false>), grd, blk, 0, st, *g, bs, nullptr, 0L, fo);
     });
     return launch_status("ssg2d stress");
 }
 
 int launch_ssg2d_sweeps_saved(const sw_ssg2d *g, int nbatch, long bs, bool stress, float *sv, long ss,
-                              hipStream_t st)
+                              const Fold2D &fo, hipStream_t st)
 {
     const dim3 blk(64, 4, 1);
     SW_REQUIRE(g->scheme == SW_2D_VS, "saved sweeps exist for the VS scheme only");
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
         if (stress)
-            hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS, true>), grd, blk, 0, st, *g, bs, sv, ss);
+            hipLaunchKernelGGL((k_stress2d<N, SW_2D_VS, true>), grd, blk, 0, st, *g, bs, sv, ss, fo);
         else
-            hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS, true>), grd, blk, 0, st, *g, bs, sv, ss);
+            hipLaunchKernelGGL((k_velocity2d<N, SW_2D_VS, true>), grd, blk, 0, st, *g, bs, sv, ss, fo);
     });
     return launch_status(stress ? "ssg2d stress (saved)" : "ssg2d velocity (saved)");
 }
 
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st)
 {
-    return launch_ssg2d_velocity_batch(g, 1, 0, st);
+    return launch_ssg2d_velocity_batch(g, 1, 0, Fold2D{}, st);
 }
 
 int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st)
 {
-    return launch_ssg2d_stress_batch(g, 1, 0, st);
+    return launch_ssg2d_stress_batch(g, 1, 0, Fold2D{}, st);
 }
 
 }  // namespace sw

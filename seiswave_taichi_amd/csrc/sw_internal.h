@@ -104,6 +104,56 @@ __device__ __forceinline__ void shift_adj(sw_adj2d &q, long off)
     q.g_b1 += off; q.g_b4 += off; q.g_lamh += off; q.g_muh += off; q.g_muz += off;
 }
 
+// Point sources of the 2-D step at cell p, shared by the injection kernel and
+// the sweeps that fold the injection in (the same expressions, so the same
+// rounded values): a force (src/Elastic2D_CPML.py:170, :245) adds
+// dt*amp/rho[p] to a velocity after the velocity sweep; an explosive or
+// moment-tensor source adds amp*(-M) to the stresses before it (the 2-D
+// moment tensor is mt[0], mt[2], mt[4] = Mxx, Mzz, Mxz).
+__device__ __forceinline__ void force_src2d(sw_ssg2d &g, int kind, float amp, long p)
+{
+    if (kind == SW_SRC_FORCE_VZ)
+        g.vz[p] += g.dt * amp / g.rho[p];
+    else
+        g.vx[p] += g.dt * amp / g.rho[p];
+}
+
+__device__ __forceinline__ void stress_src2d(sw_ssg2d &g, int kind, float m0, float m2, float m4, float amp, long p)
+{
+    if (kind == SW_SRC_MOMENT) {
+        g.sxx[p] += amp * (-m0);
+        g.szz[p] += amp * (-m2);
+        g.sxz[p] += amp * (-m4);
+    } else {
+        const float e = amp * (-0.70710678118654752f);
+        g.sxx[p] += e;
+        g.szz[p] += e;
+    }
+}
+
+// Work of a 2-D step folded into its two sweeps (on small grids the separate
+// one-thread and one-block launches cost a quarter of a step):
+//  * force: a force source of this step, added by the velocity-sweep thread
+//    of the source cell after its own update (no other thread of that sweep
+//    touches the cell);
+//  * stress: an explosive / moment source of the NEXT step, added by the
+//    stress-sweep thread of the source cell after its own update (no other
+//    thread of that sweep reads stresses), i.e. right before that step;
+//  * record: the velocity receivers of this step, gathered by the first
+//    block of each shot in the stress sweep (velocities are final by then).
+// Shot b's source cell is (ix[b], iz[b]); its traces start b * ds floats
+// after rec.data.  A zero field switches a part off.
+struct Fold2D {
+    int force, it_force;
+    int stress, it_next;
+    const float *amp;
+    float m0, m2, m4;
+    int ix[SW_MAX_BATCH], iz[SW_MAX_BATCH];
+    int record, it_rec;
+    sw_receivers rec;
+    long ds;
+};
+
 }  // namespace sw
 
 // Dispatch on the stencil half-length N (accuracy) with a compile-time constant.

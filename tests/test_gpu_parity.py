@@ -541,3 +541,42 @@ def test_2d_traces_against_f64_oracle():
     per, glob, glob32 = _f64_report(gd, outs[0], outs[1])
     print(f"2-D f32 vs f64: global {glob:.3e}, worst live trace {per:.3e}")
     assert glob == glob32 and glob <= 1e-5 and per <= TOL, (glob, per)
+
+
+@pytest.mark.parametrize("case", ["stress_receivers", "halo_source", "folded"])
+def test_2d_injection_and_gather_paths_match_oracle(case):
+    """The 2-D sweeps fold the source injection and the velocity-receiver
+    gather into themselves where they can; the separate launches remain for
+    the rest.  Explosive source, bit for bit against the oracle: both folded
+    (velocity receivers), a gather that records a stress (no folding of the
+    next step's source into the stress sweep), and a source in the stencil
+    halo (no sweep thread owns it)."""
+    from seiswave_taichi_amd.Elastic2D_CPML import ElasticWAVE
+    n, steps, f, dt, d = (90, 70), 150, 20.0, 8e-4, (10.0, 10.0)
+    vs, vp, rho = layered_model(n, seed=12)
+    cell = (1, 35) if case == "halo_source" else (45, 35)
+    comps = ("vx", "sxx", "vz") if case == "stress_receivers" else ("vx", "vz")
+    rx = np.arange(4, n[0] - 4, 7)
+    recv = np.stack([rx, np.full_like(rx, 6)], 1)
+    recv[0] = cell  # one receiver on the source cell
+    amps = ricker2(np.arange(steps), dt, 1.5 / f, f, 1.0)
+    w = ElasticWAVE(vs, vp, rho, *d, dt, *cell, 0, 0, steps, 2, f, receivers=recv, components=comps,
+                    source={"kind": "explosive", "cell": cell, "wavelet": amps})
+    p = pml_dict(float(vp.max()), 10, d, f, float(vs.max()), dims=2)
+    w.SetADEPML2D([True] * 4, p)
+    w.run("VS", 60, 0)
+    w.run("VS", steps - 60, 60)  # two calls: the state between them is the reference's
+    o = O.Oracle2D(vs, vp, rho, *d, dt, *cell, 0, 0, steps, 2, f)
+    o.set_pml([True] * 4, p)
+    ref = np.zeros((steps, len(comps), len(rx)), np.float32)
+    for it in range(steps):
+        e = np.float32(amps[it]) * np.float32(-0.70710678118654752)
+        o.fields["sxx"][cell] += e
+        o.fields["szz"][cell] += e
+        o.vs_step(it, 0.0)
+        for c, name in enumerate(comps):
+            ref[it, c] = o.fields[name][recv[:, 0], recv[:, 1]]
+    assert np.abs(ref).max() > 0
+    np.testing.assert_array_equal(w.data.to_numpy(), ref)
+    for name in o.FIELDS:
+        np.testing.assert_array_equal(getattr(w, name).to_numpy(), o.fields[name], err_msg=name)
