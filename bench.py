@@ -43,6 +43,7 @@ def unit_mt(seed):
 
 def workload(name, nt, seed=0):
     """Model, geometry and PML of a named workload (host arrays)."""
+    surface = [True] * 6
     if name in ("tensor_200", "spml_200"):
         n = 200
         shape = (n, n, n)
@@ -54,7 +55,7 @@ def workload(name, nt, seed=0):
         lin = np.round(np.linspace(3, n - 4, 50)).astype(np.int64)
         rx, ry = (a.ravel() for a in np.meshgrid(lin, lin, indexing="ij"))
         pml = pml_params(3500.0, 2000.0, npml, d, f, kmax=4.0)
-    elif name == "layered_600x600x400":
+    elif name in ("layered_600x600x400", "shard_600x600x54"):
         from seiswave_taichi_amd.models import layered_folded_model
         shape = (600, 600, 400)
         d, f, npml = (12.5, 12.5, 12.5), 10.0, 20
@@ -65,11 +66,20 @@ def workload(name, nt, seed=0):
         lin = np.round(np.linspace(3, shape[0] - 4, 200)).astype(np.int64)
         rx, ry = (a.ravel() for a in np.meshgrid(lin, lin, indexing="ij"))
         pml = pml_params(vp_max, float(vs.max()), npml, d, f)
+        if name == "shard_600x600x54":
+            # one GPU's share of the 600x600x400 grid split 8 ways along z: a
+            # middle slab of 50 planes plus the 2-plane halo on each side, CPML
+            # on the four x and y faces only (its z faces are halo exchanges)
+            k0 = 175 - 2
+            vs, vp, rho = (np.ascontiguousarray(a[:, :, k0:k0 + 54]) for a in (vs, vp, rho))
+            shape = (600, 600, 54)
+            src = (300, 300, 27)
+            surface = [True, True, True, True, False, False]
     else:
         raise ValueError(name)
     rz = np.full_like(rx, 2 + 5)
     return dict(shape=shape, vs=vs, vp=vp, rho=rho, d=d, dt=dt, f=f, src=src, rec=(rx, ry, rz), pml=pml,
-                mt=unit_mt(seed), nt=nt)
+                mt=unit_mt(seed), nt=nt, surface=surface)
 
 
 def algorithmic_bytes(shape, s, slabs, spml=False):
@@ -114,7 +124,7 @@ def cpu_baseline(w, budget_s=15.0):
     rx, ry, rz = w["rec"]
     o = O.Oracle3D("tensor", w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], w["src"],
                    np.stack([rx, ry, rz], 1), 1600, 2, freq=w["f"], mt=w["mt"])
-    o.set_pml([True] * 6, w["pml"])
+    o.set_pml(w["surface"], w["pml"])
     o.run(2)  # warm-up
     t0 = time.perf_counter()
     o.step(2)
@@ -252,7 +262,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="tensor_200",
-                    choices=["tensor_200", "spml_200", "layered_600x600x400", "gradient_800x300", "vs2d_400"])
+                    choices=["tensor_200", "spml_200", "layered_600x600x400", "shard_600x600x54", "gradient_800x300",
+                             "vs2d_400"])
     ap.add_argument("--fd-cells", type=int, default=4,
                     help="gradient_800x300: cells per parameter of the finite-difference spot check")
     ap.add_argument("--kernel-reps", type=int, default=200)
@@ -295,7 +306,7 @@ def main():
         mt3[0, 0], mt3[1, 1], mt3[2, 2], mt3[0, 1], mt3[0, 2], mt3[1, 2] = m
         sim = Tensor(w["vs"], w["vp"], w["rho"], *w["d"], w["dt"], *w["src"], rx[None], ry[None], rz[None],
                      mt3, 1.0, nt, 2, w["f"], source=src)
-    sim.SetADEPML3D([True] * 6, w["pml"])
+    sim.SetADEPML3D(w["surface"], w["pml"])
     torch.cuda.synchronize()
 
     sim.run(args.warmup, 0)
@@ -353,7 +364,7 @@ def main():
         "config": {"workload": f"{args.config}: {nx}x{ny}x{nz} Elastic3D_SSG_"
                                f"{'SPML' if spml else 'Tensor'} CPML20, order 4, moment tensor, "
                                f"{len(rx)} receivers", "grid": [nx, ny, nz], "accuracy": 2,
-                   "pml_cells": 20, "receivers": int(len(rx))},
+                   "pml_cells": 20, "pml_faces": int(sum(w["surface"])), "receivers": int(len(rx))},
         "roofline": {"bound": "hbm", "kernel": tag, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": b_dom, "launch_ms": round(kt[dom] * 1e3, 4)},
