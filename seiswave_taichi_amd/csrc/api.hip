@@ -22,6 +22,8 @@ int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st);
+int launch_ssg3d_velocity_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
+int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st);
 int launch_ssg2d_stress(const sw_ssg2d *g, hipStream_t st);
 int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
@@ -113,22 +115,10 @@ __global__ void k_inject3d(sw_ssg3d g, sw_source s, int it)
     const float amp = s.amp[it];
     const long p = ((long)s.iz * g.ny + s.iy) * g.nx + s.ix;
     switch (s.kind) {
-    case SW_SRC_FORCE_VZ: g.vz[p] += g.dt * amp / g.rho[p]; break;
-    case SW_SRC_FORCE_VX: g.vx[p] += g.dt * amp / g.rho[p]; break;
+    case SW_SRC_FORCE_VZ:
+    case SW_SRC_FORCE_VX: force_src3d(g, s.kind, amp, p); break;
     case SW_SRC_MOMENT:
-        g.sxx[p] += amp * (-s.mt[0]);
-        g.syy[p] += amp * (-s.mt[1]);
-        g.szz[p] += amp * (-s.mt[2]);
-        g.sxy[p] += amp * (-s.mt[3]);
-        g.sxz[p] += amp * (-s.mt[4]);
-        g.syz[p] += amp * (-s.mt[5]);
-        break;
-    case SW_SRC_EXPLOSIVE: {
-        const float e = amp * (-0.57735026918962576f);
-        g.sxx[p] += e;
-        g.syy[p] += e;
-        g.szz[p] += e;
-    } break;
+    case SW_SRC_EXPLOSIVE: stress_src3d(g, s.kind, s.mt, amp, p); break;
     default: break;
     }
 }
@@ -219,6 +209,35 @@ Fold2D plan_fold2d(const sw_ssg2d *g, const sw_source *src, const sw_receivers *
     fo.m0 = src->mt[0];
     fo.m2 = src->mt[2];
     fo.m4 = src->mt[4];
+    return fo;
+}
+
+// What of a 3-D step the sweeps can do themselves (Fold3D): a source cell
+// that owns a sweep thread ([N, n-N) on every axis), receivers recording
+// velocities only, and a next-step stress source only when the gather is
+// folded too and no free surface is zeroed between the steps (the reference
+// zeroes szz before it injects).
+Fold3D plan_fold3d(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, int flags)
+{
+    Fold3D fo{};
+    const int N = g->order;
+    const bool cell_ok = src->kind != SW_SRC_NONE && src->ix >= N && src->ix < g->nx - N && src->iy >= N &&
+                         src->iy < g->ny - N && src->iz >= N && src->iz < g->nz - N;
+    bool rec_ok = rec && rec->nrec > 0;
+    for (int c = 0; rec_ok && c < rec->ncomp; ++c) rec_ok = rec->comp[c] >= 0 && rec->comp[c] <= 2;
+    if (rec_ok) {
+        fo.record = 1;
+        fo.rec = *rec;
+        fo.gather = (int)std::min<long>(64, ((long)rec->nrec * rec->ncomp + 255) / 256);
+    }
+    const bool no_gather_after = !rec || rec->nrec == 0 || rec_ok;
+    if (cell_ok && force_source(src)) fo.force = src->kind;
+    if (cell_ok && stress_source(src) && no_gather_after && !(flags & SW_RUN_FREE_SURFACE)) fo.stress = src->kind;
+    fo.amp = src->amp;
+    for (int t = 0; t < 6; ++t) fo.mt[t] = src->mt[t];
+    fo.ix = src->ix;
+    fo.iy = src->iy;
+    fo.iz = src->iz;
     return fo;
 }
 
@@ -443,19 +462,22 @@ int sw_ssg3d_run_ex(const sw_ssg3d *g, const sw_source *src, const sw_receivers 
     if (rec && ((rc = check_receivers(rec, it0)) || (rc = check_receivers(rec, it0 + nsteps - 1)))) return rc;
     SW_REQUIRE((flags & ~SW_RUN_FREE_SURFACE) == 0, "unknown run flags 0x%x", flags);
     hipStream_t st = as_stream(stream);
-    const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
     const FieldPtrs fp = fields3d(g);
-    for (int it = it0; it < it0 + nsteps; ++it) {
+    const int it_end = it0 + nsteps;
+    const Fold3D plan = plan_fold3d(g, src, rec, flags);
+    Fold3D fo = plan;
+    for (int it = it0; it < it_end; ++it) {
+        fo.it_force = it;
+        fo.stress = it + 1 < it_end ? plan.stress : 0;
+        fo.it_next = it + 1;
+        fo.it_rec = it;
         if ((flags & SW_RUN_FREE_SURFACE) && (rc = free_surface3d_launch(g, st))) return rc;
-        if (pre && src->kind != SW_SRC_NONE) {
+        if (stress_source(src) && (it == it0 || !plan.stress))
             hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
-        }
-        if ((rc = spml ? launch_ssg3d_velocity_spml(g, st) : launch_ssg3d_velocity(g, st))) return rc;
-        if (!pre && src->kind != SW_SRC_NONE) {
-            hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
-        }
-        if ((rc = spml ? launch_ssg3d_stress_spml(g, st) : launch_ssg3d_stress(g, st))) return rc;
-        if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
+        if ((rc = launch_ssg3d_velocity_fold(g, fo, st))) return rc;
+        if (force_source(src) && !plan.force) hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg3d_stress_fold(g, fo, st))) return rc;
+        if (rec && !plan.record && (rc = record_launch(fp, rec, it, st))) return rc;
     }
     return launch_status("ssg3d run");
 }

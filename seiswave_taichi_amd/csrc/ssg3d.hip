@@ -69,7 +69,7 @@ struct Cell {
 
 __device__ __forceinline__ Cell box_cell(const Geo &geo)
 {
-    const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int b = xcd_logical_block(blockIdx.x, geo.nblock);
     const Box &B = geo.box;
     const int cb = b % B.ncb, zb = b / B.ncb;
     const int t = cb * kThreads + (int)threadIdx.x;
@@ -136,7 +136,7 @@ __device__ __forceinline__ float cn_at(float q0, float *dst, float n, float d, f
 
 
 template <int N, int CLS, int PM>
-__global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
+__global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold3D fo)
 {
     const Cell cl = box_cell(geo);
     const int i = cl.i, j = cl.j;
@@ -310,6 +310,10 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo)
         P = Q;
 #endif
     }
+    // folded force source: after the loop (no other thread of this sweep
+    // reads the cell's velocity), which keeps it out of the loop's registers
+    if (fo.force && i == fo.ix && j == fo.iy && fo.iz >= cl.k0 && fo.iz < cl.k1)
+        force_src3d(g, fo.force, fo.amp[fo.it_force], ((long)fo.iz * ny + j) * nx + i);
 }
 
 // -------------------------------------------------------------- stress sweep
@@ -370,9 +374,27 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
     }
 }
 
-template <int N, int CLS, int PM>
-__global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
+// Velocity receivers of this step, gathered by the nb blocks appended to the
+// first stress launch: data[(it*ncomp + comp)*nrec + r].
+__device__ void gather3d(const sw_ssg3d &g, const Fold3D &fo, int b, int nb)
 {
+    const sw_receivers &r = fo.rec;
+    const int n = r.nrec * r.ncomp;
+    float *out = r.data + (long)fo.it_rec * n;
+    for (int t = b * kThreads + (int)threadIdx.x; t < n; t += nb * kThreads) {
+        const int comp = t / r.nrec, q = t - comp * r.nrec;
+        const int code = r.comp[comp];
+        out[t] = (code == 0 ? g.vx : code == 1 ? g.vy : g.vz)[r.idx[q]];
+    }
+}
+
+template <int N, int CLS, int PM>
+__global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D fo)
+{
+    if ((int)blockIdx.x >= geo.nblock) {
+        gather3d(g, fo, (int)blockIdx.x - geo.nblock, (int)gridDim.x - geo.nblock);
+        return;
+    }
     const Cell cl = box_cell(geo);
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -549,6 +571,10 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo)
         P = Q;
 #endif
     }
+    // folded next-step stress source, after the loop (no other thread of
+    // this sweep reads stresses)
+    if (fo.stress && i == fo.ix && j == fo.iy && fo.iz >= cl.k0 && fo.iz < cl.k1)
+        stress_src3d(g, fo.stress, fo.mt, fo.amp[fo.it_next], ((long)fo.iz * ny + j) * nx + i);
 }
 
 #ifndef SW_CZ0
@@ -615,65 +641,72 @@ Plan make_plan(const sw_ssg3d &g, int N)
 }
 
 template <int N, int CLS>
-void launch_velocity(const sw_ssg3d &g, const Plan &p, hipStream_t st)
+void launch_velocity(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
 {
     for (int t = 0; t < p.n; ++t) {
         const dim3 grd(p.g[t].nblock), blk(kThreads);
         if (p.pm[t] == 1)
-            hipLaunchKernelGGL((k_velocity<N, CLS, 1>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_velocity<N, CLS, 1>), grd, blk, 0, st, g, p.g[t], fo);
         else if (p.pm[t] == 3)
-            hipLaunchKernelGGL((k_velocity<N, CLS, 3>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_velocity<N, CLS, 3>), grd, blk, 0, st, g, p.g[t], fo);
         else
-            hipLaunchKernelGGL((k_velocity<N, CLS, 7>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_velocity<N, CLS, 7>), grd, blk, 0, st, g, p.g[t], fo);
     }
 }
 
+// The first launch also carries the folded receiver gather (fo.gather extra
+// blocks); the others get a copy without it.
 template <int N, int CLS>
-void launch_stress(const sw_ssg3d &g, const Plan &p, hipStream_t st)
+void launch_stress(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
 {
+    Fold3D rest = fo;
+    rest.record = 0;
+    rest.gather = 0;
     for (int t = 0; t < p.n; ++t) {
-        const dim3 grd(p.g[t].nblock), blk(kThreads);
+        const Fold3D &f = t == 0 ? fo : rest;
+        const dim3 grd(p.g[t].nblock + (f.record ? f.gather : 0)), blk(kThreads);
         if (p.pm[t] == 1)
-            hipLaunchKernelGGL((k_stress<N, CLS, 1>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, CLS, 1>), grd, blk, 0, st, g, p.g[t], f);
         else if (p.pm[t] == 3)
-            hipLaunchKernelGGL((k_stress<N, CLS, 3>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, CLS, 3>), grd, blk, 0, st, g, p.g[t], f);
         else
-            hipLaunchKernelGGL((k_stress<N, CLS, 7>), grd, blk, 0, st, g, p.g[t]);
+            hipLaunchKernelGGL((k_stress<N, CLS, 7>), grd, blk, 0, st, g, p.g[t], f);
     }
 }
 
 }  // namespace
 
-int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st)
+int launch_ssg3d_velocity_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st)
 {
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
         if (g->cls == SW_CLASS_TENSOR)
-            launch_velocity<N, SW_CLASS_TENSOR>(*g, p, st);
+            launch_velocity<N, SW_CLASS_TENSOR>(*g, p, fo, st);
+        else if (g->cls == SW_CLASS_SPML)
+            launch_velocity<N, SW_CLASS_SPML>(*g, p, fo, st);
         else
-            launch_velocity<N, SW_CLASS_CPML>(*g, p, st);
+            launch_velocity<N, SW_CLASS_CPML>(*g, p, fo, st);
     });
-    return launch_status("ssg3d velocity");
-}
-
-int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st)
-{
-    SW_DISPATCH_ORDER(g->order, { launch_stress<N, SW_CLASS_CPML>(*g, make_plan(*g, N), st); });
-    return launch_status("ssg3d stress");
+    return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d velocity (split-field PML)" : "ssg3d velocity");
 }
 
 // Split-field PML class (src/Elastic3D_SSG_SPML.py:160-393): the same plan,
 // the axes' lo/hi bounding their split-free ranges.
-int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st)
+int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st)
 {
-    SW_DISPATCH_ORDER(g->order, { launch_velocity<N, SW_CLASS_SPML>(*g, make_plan(*g, N), st); });
-    return launch_status("ssg3d velocity (split-field PML)");
+    SW_DISPATCH_ORDER(g->order, {
+        const Plan p = make_plan(*g, N);
+        if (g->cls == SW_CLASS_SPML)
+            launch_stress<N, SW_CLASS_SPML>(*g, p, fo, st);
+        else
+            launch_stress<N, SW_CLASS_CPML>(*g, p, fo, st);
+    });
+    return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d stress (split-field PML)" : "ssg3d stress");
 }
 
-int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st)
-{
-    SW_DISPATCH_ORDER(g->order, { launch_stress<N, SW_CLASS_SPML>(*g, make_plan(*g, N), st); });
-    return launch_status("ssg3d stress (split-field PML)");
-}
+int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_velocity_fold(g, Fold3D{}, st); }
+int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_stress_fold(g, Fold3D{}, st); }
+int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_velocity_fold(g, Fold3D{}, st); }
+int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_stress_fold(g, Fold3D{}, st); }
 
 }  // namespace sw

@@ -104,6 +104,52 @@ __device__ __forceinline__ void shift_adj(sw_adj2d &q, long off)
     q.g_b1 += off; q.g_b4 += off; q.g_lamh += off; q.g_muh += off; q.g_muz += off;
 }
 
+// Point sources of the 3-D step at cell p, shared by the injection kernel and
+// the sweeps that fold the injection in: a force adds dt*amp/rho[p] to a
+// velocity after the velocity sweep (src/Elastic3D_SSG_CPML.py:245,
+// src/Elastic3D_SSG_SPML.py:294); a moment tensor adds amp*(-M_ab) to the six
+// stresses before it (src/Elastic3D_SSG_Tensor.py:186-193), an explosive
+// source amp*(-1/sqrt(3)) to the normal ones.
+__device__ __forceinline__ void force_src3d(const sw_ssg3d &g, int kind, float amp, long p)
+{
+    if (kind == SW_SRC_FORCE_VZ)
+        g.vz[p] += g.dt * amp / g.rho[p];
+    else
+        g.vx[p] += g.dt * amp / g.rho[p];
+}
+
+__device__ __forceinline__ void stress_src3d(const sw_ssg3d &g, int kind, const float *mt, float amp, long p)
+{
+    if (kind == SW_SRC_MOMENT) {
+        g.sxx[p] += amp * (-mt[0]);
+        g.syy[p] += amp * (-mt[1]);
+        g.szz[p] += amp * (-mt[2]);
+        g.sxy[p] += amp * (-mt[3]);
+        g.sxz[p] += amp * (-mt[4]);
+        g.syz[p] += amp * (-mt[5]);
+    } else {
+        const float e = amp * (-0.57735026918962576f);
+        g.sxx[p] += e;
+        g.syy[p] += e;
+        g.szz[p] += e;
+    }
+}
+
+// Work of a 3-D step folded into its sweeps, as Fold2D: a force source of
+// this step (velocity-sweep thread of the source cell, after its update), a
+// moment / explosive source of the NEXT step (stress-sweep thread of the
+// source cell, after its update) and the velocity receivers of this step
+// (gathered by `gather` extra blocks appended to the first stress launch).
+struct Fold3D {
+    int force, it_force;
+    int stress, it_next;
+    const float *amp;
+    float mt[6];
+    int ix, iy, iz;
+    int record, it_rec, gather;
+    sw_receivers rec;
+};
+
 // Point sources of the 2-D step at cell p, shared by the injection kernel and
 // the sweeps that fold the injection in (the same expressions, so the same
 // rounded values): a force (src/Elastic2D_CPML.py:170, :245) adds
