@@ -2,6 +2,7 @@
 // gather, multi-step drivers and the free-surface helpers.
 #include "sw_internal.h"
 
+#include <climits>
 #include <cstring>
 
 namespace sw {
@@ -72,7 +73,19 @@ int check_ssg3d(const sw_ssg3d *g, bool spml)
     SW_REQUIRE(g->nx >= need && g->ny >= need && g->nz >= need,
                "grid %dx%dx%d too small for order %d (each axis needs >= %d)", g->nx, g->ny, g->nz,
                g->order, need);
-    SW_REQUIRE((long)g->nx * g->ny * g->nz < (1L << 40), "grid too large");
+    // The sweeps address a z chunk (at most 64 planes) and its stencil halo
+    // with 32-bit byte offsets from pointers rebased per block, the z memory
+    // slab with 32-bit offsets from its base, and receivers by int32 cells.
+    const long cells = (long)g->nx * g->ny * g->nz, plane = 4L * g->nx * g->ny;
+    SW_REQUIRE(cells <= INT32_MAX, "grid %dx%dx%d: %ld cells, above the 2^31-1 an int32 cell index reaches",
+               g->nx, g->ny, g->nz, cells);
+    SW_REQUIRE((64L + 2 * g->order + 1) * plane <= (1L << 32),
+               "planes of %dx%d cells too large: a z chunk of the sweeps must stay below 4 GiB", g->nx, g->ny);
+    // the six memory variables of each axis share one 32-bit offset range
+    const long wx = g->px.lo + g->nx - g->px.hi, wy = g->py.lo + g->ny - g->py.hi, wz = g->pz.lo + g->nz - g->pz.hi;
+    SW_REQUIRE(24L * wx * g->ny * g->nz <= (1L << 32) && 24L * wy * g->nx * g->nz <= (1L << 32) &&
+                   24L * wz * g->nx * g->ny <= (1L << 32),
+               "CPML memory slabs of the %dx%dx%d grid above 4 GiB per axis", g->nx, g->ny, g->nz);
     SW_REQUIRE(g->rho && g->lam && g->mu, "model arrays missing");
     SW_REQUIRE(g->vx && g->vy && g->vz && g->sxx && g->syy && g->szz && g->sxy && g->sxz && g->syz,
                "wavefield arrays missing");

@@ -135,7 +135,10 @@ __device__ __forceinline__ float cn_at(float q0, float *dst, float n, float d, f
 }
 
 
-template <int N, int CLS, int PM>
+// LARGE: arrays above 4 GiB; the block rebases its full-grid pointers to its
+// first plane (otherwise kb is 0 and the rebasing compiles away, which keeps
+// the scalar registers of the common case free).
+template <int N, int CLS, int PM, bool LARGE>
 __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold3D fo)
 {
     const Cell cl = box_cell(geo);
@@ -158,7 +161,8 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
     SpCol sp{};
     if (SPLIT) sp = spml_col<PM>(g, i, j);
     const long NC = (long)nx * ny * nz;
-    const Res R = make_res(g);
+    const int kb = LARGE ? cl.k0 - N : 0;  // base plane of this block's offsets
+    const Res R = make_res(g, kb);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
     const Coef cy = coef_of(g.py, j, sl.ys >= 0);
     const uint32_t pc = 4u * (uint32_t)(j * nx + i);
@@ -167,25 +171,25 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
     const int k0 = cl.k0;
 #pragma unroll
     for (int t = 0; t < 2 * N - 1; ++t) {
-        wxz[t] = ld(R.sxz, pc + (uint32_t)(k0 - N + t) * SZ);
-        wyz[t] = ld(R.syz, pc + (uint32_t)(k0 - N + t) * SZ);
-        wzz[t] = ld(R.szz, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
+        wxz[t] = ld(R.sxz, pc + (uint32_t)(k0 - N + t - kb) * SZ);
+        wyz[t] = ld(R.syz, pc + (uint32_t)(k0 - N + t - kb) * SZ);
+        wzz[t] = ld(R.szz, pc + (uint32_t)(k0 - N + 1 + t - kb) * SZ);
     }
     VPlane<N> P;
     const long cell0 = (long)j * nx + i, plane = (long)nx * ny;
 #if SW_PREFETCH
-    load_vplane<N, CPM>(P, g, R, pc, k0, SY, SZ, sl);
+    load_vplane<N, CPM>(P, g, R, pc, k0, kb, SY, SZ, sl);
     if (SW_SPLIT_PREFETCH && SPLIT) load_split<9, PM>(P.q, g, sp, cell0 + k0 * plane, k0, 0);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         VPlane<N> Q;
         if (k + 1 < cl.k1) {
-            load_vplane<N, CPM>(Q, g, R, pc, k + 1, SY, SZ, sl);
+            load_vplane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
             if (SW_SPLIT_PREFETCH && SPLIT) load_split<9, PM>(Q.q, g, sp, cell0 + (k + 1) * plane, k + 1, 0);
         }
 #else
-        load_vplane<N, CPM>(P, g, R, pc, k, SY, SZ, sl);
+        load_vplane<N, CPM>(P, g, R, pc, k, kb, SY, SZ, sl);
 #endif
         // split-field PML: flags and damping of this plane, split arrays at (i, j, k)
         const bool zi = SPLIT && (PM & 4) ? g.pz.in[k] != 0 : false;
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
             for (int t = 0; t < 9; ++t) qv[t] = qp[(long)t * NC];
         }
 #endif
-        const uint32_t p = pc + (uint32_t)k * SZ;
+        const uint32_t p = pc + (uint32_t)(k - kb) * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
         const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
         wxz[2 * N - 1] = P.wxz;
@@ -335,10 +339,10 @@ struct SPlane {
 };
 
 template <int N, int PM>
-__device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
-                                            const Slab &sl)
+__device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int kb,
+                                            int SY, uint32_t SZ, const Slab &sl)
 {
-    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
+    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)(k - kb) * SZ;
     const int c0 = N * 4 + N * SY;
     xtaps_bm<N>(P.vx_x, R.vx, vb, c0);
     taps_fm<N>(P.vx_y, R.vx, vb, c0, SY);
@@ -388,7 +392,7 @@ __device__ void gather3d(const sw_ssg3d &g, const Fold3D &fo, int b, int nb)
     }
 }
 
-template <int N, int CLS, int PM>
+template <int N, int CLS, int PM, bool LARGE>
 __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D fo)
 {
     if ((int)blockIdx.x >= geo.nblock) {
@@ -415,7 +419,8 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
     SpCol sp{};
     if (SPLIT) sp = spml_col<PM>(g, i, j);
     const long NC = (long)nx * ny * nz;
-    const Res R = make_res(g);
+    const int kb = LARGE ? cl.k0 - N : 0;  // base plane of this block's offsets
+    const Res R = make_res(g, kb);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
     const Coef cy = coef_of(g.py, j, sl.ys >= 0);
     const uint32_t pc = 4u * (uint32_t)(j * nx + i);
@@ -424,20 +429,20 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
     const int k0 = cl.k0;
 #pragma unroll
     for (int t = 0; t < 2 * N - 1; ++t) {
-        wvz[t] = ld(R.vz, pc + (uint32_t)(k0 - N + t) * SZ);
-        wvx[t] = ld(R.vx, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
-        wvy[t] = ld(R.vy, pc + (uint32_t)(k0 - N + 1 + t) * SZ);
+        wvz[t] = ld(R.vz, pc + (uint32_t)(k0 - N + t - kb) * SZ);
+        wvx[t] = ld(R.vx, pc + (uint32_t)(k0 - N + 1 + t - kb) * SZ);
+        wvy[t] = ld(R.vy, pc + (uint32_t)(k0 - N + 1 + t - kb) * SZ);
     }
     SPlane<N> P;
 #if SW_PREFETCH
-    load_splane<N, CPM>(P, g, R, pc, k0, SY, SZ, sl);
+    load_splane<N, CPM>(P, g, R, pc, k0, kb, SY, SZ, sl);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         SPlane<N> Q;
-        if (k + 1 < cl.k1) load_splane<N, CPM>(Q, g, R, pc, k + 1, SY, SZ, sl);
+        if (k + 1 < cl.k1) load_splane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
 #else
-        load_splane<N, CPM>(P, g, R, pc, k, SY, SZ, sl);
+        load_splane<N, CPM>(P, g, R, pc, k, kb, SY, SZ, sl);
 #endif
         const bool zi = SPLIT && (PM & 4) ? g.pz.in[k] != 0 : false;
         const bool zh = SPLIT && (PM & 4) ? g.pz.in_h[k] != 0 : false;
@@ -452,7 +457,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
 #pragma unroll
             for (int t = 0; t < 15; ++t) qv[t] = qp[(long)(9 + t) * NC];
         }
-        const uint32_t p = pc + (uint32_t)k * SZ;
+        const uint32_t p = pc + (uint32_t)(k - kb) * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
         const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
         wvz[2 * N - 1] = P.wvz;
@@ -640,23 +645,23 @@ Plan make_plan(const sw_ssg3d &g, int N)
     return p;
 }
 
-template <int N, int CLS>
+template <int N, int CLS, bool LARGE>
 void launch_velocity(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
 {
     for (int t = 0; t < p.n; ++t) {
         const dim3 grd(p.g[t].nblock), blk(kThreads);
         if (p.pm[t] == 1)
-            hipLaunchKernelGGL((k_velocity<N, CLS, 1>), grd, blk, 0, st, g, p.g[t], fo);
+            hipLaunchKernelGGL((k_velocity<N, CLS, 1, LARGE>), grd, blk, 0, st, g, p.g[t], fo);
         else if (p.pm[t] == 3)
-            hipLaunchKernelGGL((k_velocity<N, CLS, 3>), grd, blk, 0, st, g, p.g[t], fo);
+            hipLaunchKernelGGL((k_velocity<N, CLS, 3, LARGE>), grd, blk, 0, st, g, p.g[t], fo);
         else
-            hipLaunchKernelGGL((k_velocity<N, CLS, 7>), grd, blk, 0, st, g, p.g[t], fo);
+            hipLaunchKernelGGL((k_velocity<N, CLS, 7, LARGE>), grd, blk, 0, st, g, p.g[t], fo);
     }
 }
 
 // The first launch also carries the folded receiver gather (fo.gather extra
 // blocks); the others get a copy without it.
-template <int N, int CLS>
+template <int N, int CLS, bool LARGE>
 void launch_stress(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
 {
     Fold3D rest = fo;
@@ -666,26 +671,47 @@ void launch_stress(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream
         const Fold3D &f = t == 0 ? fo : rest;
         const dim3 grd(p.g[t].nblock + (f.record ? f.gather : 0)), blk(kThreads);
         if (p.pm[t] == 1)
-            hipLaunchKernelGGL((k_stress<N, CLS, 1>), grd, blk, 0, st, g, p.g[t], f);
+            hipLaunchKernelGGL((k_stress<N, CLS, 1, LARGE>), grd, blk, 0, st, g, p.g[t], f);
         else if (p.pm[t] == 3)
-            hipLaunchKernelGGL((k_stress<N, CLS, 3>), grd, blk, 0, st, g, p.g[t], f);
+            hipLaunchKernelGGL((k_stress<N, CLS, 3, LARGE>), grd, blk, 0, st, g, p.g[t], f);
         else
-            hipLaunchKernelGGL((k_stress<N, CLS, 7>), grd, blk, 0, st, g, p.g[t], f);
+            hipLaunchKernelGGL((k_stress<N, CLS, 7, LARGE>), grd, blk, 0, st, g, p.g[t], f);
     }
 }
 
 }  // namespace
+
+// Arrays above 4 GiB take the rebasing kernels.
+inline bool large_grid(const sw_ssg3d &g) { return 4L * g.nx * g.ny * g.nz > (1L << 32); }
+
+template <int N, int CLS>
+void launch_velocity_any(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
+{
+    if (large_grid(g))
+        launch_velocity<N, CLS, true>(g, p, fo, st);
+    else
+        launch_velocity<N, CLS, false>(g, p, fo, st);
+}
+
+template <int N, int CLS>
+void launch_stress_any(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
+{
+    if (large_grid(g))
+        launch_stress<N, CLS, true>(g, p, fo, st);
+    else
+        launch_stress<N, CLS, false>(g, p, fo, st);
+}
 
 int launch_ssg3d_velocity_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st)
 {
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
         if (g->cls == SW_CLASS_TENSOR)
-            launch_velocity<N, SW_CLASS_TENSOR>(*g, p, fo, st);
+            launch_velocity_any<N, SW_CLASS_TENSOR>(*g, p, fo, st);
         else if (g->cls == SW_CLASS_SPML)
-            launch_velocity<N, SW_CLASS_SPML>(*g, p, fo, st);
+            launch_velocity_any<N, SW_CLASS_SPML>(*g, p, fo, st);
         else
-            launch_velocity<N, SW_CLASS_CPML>(*g, p, fo, st);
+            launch_velocity_any<N, SW_CLASS_CPML>(*g, p, fo, st);
     });
     return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d velocity (split-field PML)" : "ssg3d velocity");
 }
@@ -697,9 +723,9 @@ int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
         if (g->cls == SW_CLASS_SPML)
-            launch_stress<N, SW_CLASS_SPML>(*g, p, fo, st);
+            launch_stress_any<N, SW_CLASS_SPML>(*g, p, fo, st);
         else
-            launch_stress<N, SW_CLASS_CPML>(*g, p, fo, st);
+            launch_stress_any<N, SW_CLASS_CPML>(*g, p, fo, st);
     });
     return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d stress (split-field PML)" : "ssg3d stress");
 }

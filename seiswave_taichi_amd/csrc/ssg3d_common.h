@@ -21,7 +21,10 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb)
 
 // Fields are addressed by 32-bit byte offsets from their base pointers: one
 // offset register per column serves every field, and the wave-uniform tap
-// deltas fold into the load instructions.  The host keeps every array below
+// deltas fold into the load instructions.  A block rebases the full-grid
+// pointers to the first plane its z chunk reads (make_res), so their offsets
+// only span the chunk's planes plus the stencil halo, whatever the array
+// size; the host checks that this span, and each memory slab, stays below
 // 4 GiB.
 __device__ __forceinline__ float ld(const float *__restrict__ base, uint32_t off)
 {
@@ -45,13 +48,17 @@ struct Res {
     const float *rho, *lam, *mu;
 };
 
-__device__ __forceinline__ Res make_res(const sw_ssg3d &g)
+// Full-grid pointers rebased to plane kb.  The memory slabs keep their own
+// base (their 32-bit offsets span a whole slab, which the host keeps below
+// 4 GiB).
+__device__ __forceinline__ Res make_res(const sw_ssg3d &g, int kb)
 {
+    const long pl = (long)kb * g.nx * g.ny;
     Res r;
-    r.vx = g.vx; r.vy = g.vy; r.vz = g.vz;
-    r.sxx = g.sxx; r.syy = g.syy; r.szz = g.szz;
-    r.sxy = g.sxy; r.sxz = g.sxz; r.syz = g.syz;
-    r.rho = g.rho; r.lam = g.lam; r.mu = g.mu;
+    r.vx = g.vx + pl; r.vy = g.vy + pl; r.vz = g.vz + pl;
+    r.sxx = g.sxx + pl; r.syy = g.syy + pl; r.szz = g.szz + pl;
+    r.sxy = g.sxy + pl; r.sxz = g.sxz + pl; r.syz = g.syz + pl;
+    r.rho = g.rho + pl; r.lam = g.lam + pl; r.mu = g.mu + pl;
     r.mx = g.mem_x; r.my = g.mem_y; r.mz = g.mem_z;
     return r;
 }
@@ -215,13 +222,14 @@ struct VPlane {
     int zs;
 };
 
-// p: byte offset of (i, j, k); pc: of (i, j, 0); SY, SZ: byte strides.
+// pc: byte offset of (i, j) in a plane; SY, SZ: byte strides; R rebased to
+// plane kb.
 template <int N, int PM>
-__device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int SY, uint32_t SZ,
-                                            const Slab &sl)
+__device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, const Res &R, uint32_t pc, int k, int kb,
+                                            int SY, uint32_t SZ, const Slab &sl)
 {
     // vb: byte offset of (i-N, j-N, k); every load below is vb + a uniform delta
-    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)k * SZ;
+    const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)(k - kb) * SZ;
     const int c0 = N * 4 + N * SY;
     xtaps_fm<N>(P.sxx, R.sxx, vb, c0);
     xtaps_bm<N>(P.sxy_x, R.sxy, vb, c0);

@@ -298,3 +298,32 @@ def test_taichi_shim_covers_the_driver_calls():
     with pytest.raises(ValueError):
         vp.from_numpy(np.zeros((4, 5)))
     ti.sync()
+
+
+def test_grid_size_limits_are_checked():
+    """The 3-D sweeps address each z chunk with 32-bit offsets and receivers
+    with int32 cell indices: grids beyond those ranges are refused with an
+    error before anything is launched (no silent wrap-around)."""
+    import ctypes as C
+    lib = L.load()
+    src = L.SwSource()
+    src.kind = L.SW_SRC_NONE
+
+    def run(nx, ny, nz, lo=6):
+        g = _fake_3d()
+        g.nx, g.ny, g.nz = nx, ny, nz
+        for ax, n in ((g.px, nx), (g.py, ny), (g.pz, nz)):
+            ax.lo, ax.hi = lo, n - lo
+        rc = lib.sw_ssg3d_run_ex(C.byref(g), C.byref(src), None, 0, 1, 0x10, None)
+        return rc, L.last_error()
+
+    rc, msg = run(1300, 1300, 1300)  # 2.2e9 cells
+    assert rc == -1 and "int32 cell index" in msg
+    rc, msg = run(4000, 4000, 40)  # 16e6-cell planes: a 69-plane chunk exceeds 4 GiB
+    assert rc == -1 and "z chunk" in msg
+    rc, msg = run(200, 3000, 3000, lo=400)  # 800-cell x slabs: 24*800*3000*3000 B
+    assert rc == -1 and "memory slabs" in msg
+    # 1100 x 1000 x 1000 (4.4 GB arrays) passes the size checks and fails only
+    # on the next one (the unknown run flag)
+    rc, msg = run(1100, 1000, 1000)
+    assert rc == -1 and "unknown run flags" in msg
