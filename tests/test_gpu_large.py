@@ -9,7 +9,6 @@ import numpy as np
 import pytest
 import torch
 
-from seiswave_taichi_amd.coeffs import ricker2
 from tests.cases import pml_dict, unit_mt
 
 pytestmark = pytest.mark.gpu

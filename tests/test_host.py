@@ -5,7 +5,6 @@ import re
 
 import numpy as np
 import pytest
-import torch
 
 from oracle import oracle as O
 from seiswave_taichi_amd import _lib as L
