@@ -223,8 +223,8 @@ int sw_ssg2d_model_gradient(const sw_ssg2d *g, const sw_adj2d *adj, const float 
  * floats starting at g->vx (vx vz sxx szz sxz, then the 10 memory planes at
  * g->mem); the call advances it to step it0+n (source injection, velocity
  * and stress sweeps of src/Elastic2D_CPML.py:127-200 per step), keeping in
- * work (n*5*nx*nz floats, caller-owned) what the model gradient needs of
- * every step: the CPML-corrected derivative sums the two sweeps multiply
+ * work ((n*5 + 4)*nx*nz floats, caller-owned; the last 4 planes are scratch)
+ * what the model gradient needs of every step: the CPML-corrected derivative sums the two sweeps multiply
  * (vx and vz cells of the velocity sweep, dvx/dx, dvz/dz and dvx/dz + dvz/dx
  * of the stress sweep).  Then, for it = it0+n-1 down to it0, it does what
  * sw_ssg2d_adjoint_inject(residual, it), sw_ssg2d_adjoint_stress (state after
@@ -249,7 +249,7 @@ int sw_ssg2d_adjoint_segment(const sw_ssg2d *g, const sw_source *src, const sw_a
 int sw_ssg2d_run_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
                        const int32_t *cells, const sw_receivers *rec, int64_t data_stride, int32_t it0,
                        int32_t nsteps, void *stream);
-/* sw_ssg2d_adjoint_segment for every shot; work holds n*nbatch*5*nx*nz floats */
+/* sw_ssg2d_adjoint_segment for every shot; work holds (n*5 + 4)*nbatch*nx*nz floats */
 int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t state_stride, const sw_source *src,
                                    const int32_t *cells, const sw_adj2d *adj, int64_t adj_stride,
                                    const sw_receivers *residual, int64_t data_stride, float *work, int32_t it0,

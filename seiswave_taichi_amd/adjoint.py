@@ -112,10 +112,12 @@ class Elastic2DModeling:
         shape = (NSTATE, self.nz, self.nx) if n is None else (n, NSTATE, self.nz, self.nx)
         return torch.zeros(shape, dtype=torch.float32, device=self.device)
 
-    def _segment_buffer(self, n):
+    def _segment_buffer(self, steps, shots=1):
         """Work of one adjoint segment call: NSAVED planes per shot and step
-        (the forward derivative sums the model gradient reads)."""
-        return torch.empty((n, NSAVED, self.nz, self.nx), dtype=torch.float32, device=self.device)
+        (the forward derivative sums the model gradient reads) and 4 scratch
+        planes per shot."""
+        P = self.nz * self.nx
+        return torch.empty((steps * shots * NSAVED + 4 * shots) * P, dtype=torch.float32, device=self.device)
 
     def desc(self, state: torch.Tensor) -> L.SwSSG2D:
         g = L.SwSSG2D()
@@ -240,7 +242,7 @@ class Elastic2DModeling:
             q = self._adj_desc(adjb[0, :NSTATE], adjb[0, NSTATE:NSTATE + 4], adjb[0, NSTATE + 4:],
                                g_src[group[0]:] if self.force else None)
             work = self.new_state(nb)
-            seg = self._segment_buffer(K * nb)
+            seg = self._segment_buffer(K, nb)
             g = self.desc(work[0])
             src = self.source(self.shots[group[0]])
             res = grad_data[group[0]:group[-1] + 1].detach().to(torch.float32).contiguous()

@@ -39,17 +39,13 @@ __device__ __forceinline__ float fwd_cpml(float D, float m, float a, float b, fl
 // SAVED: the forward derivatives come from the planes the saved stress sweep
 // wrote (sv + blockIdx.z * ss, planes 2-4) instead of being recomputed from
 // the forward state; they are the same values, so the result is the same.
+// The four stage passes of one cell (i, j) of one shot (f, q, sv already at
+// the shot's offsets), writing / reading the pass-A outputs at tmp (4 planes).
 template <int N, bool SAVED>
-__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, long fs, long as, const float *sv,
-                                                      long ss)
+__device__ __forceinline__ void stress_a_cell(const sw_ssg2d &f, const sw_adj2d &q, const float *sv, float *tmp, int i,
+                                              int j)
 {
-    shift_state(f, (long)blockIdx.z * fs);
-    shift_adj(q, (long)blockIdx.z * as);
-    if (SAVED) sv += (long)blockIdx.z * ss;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
-    if (i >= nx || j >= nz) return;
     const long SZ = nx, NC = (long)nx * nz;
     const long p = (long)j * SZ + i;
     const float dx = f.dx, dz = f.dz, dt = f.dt;
@@ -115,10 +111,10 @@ __global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, lo
         t2 = bx / dx;
         t3 = bz / dz;
     }
-    q.tmp[0 * NC + p] = t0;
-    q.tmp[1 * NC + p] = t1;
-    q.tmp[2 * NC + p] = t2;
-    q.tmp[3 * NC + p] = t3;
+    tmp[0 * NC + p] = t0;
+    tmp[1 * NC + p] = t1;
+    tmp[2 * NC + p] = t2;
+    tmp[3 * NC + p] = t3;
 }
 
 __device__ __forceinline__ float tap(const float *t, long idx, long n)
@@ -127,18 +123,13 @@ __device__ __forceinline__ float tap(const float *t, long idx, long n)
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q, long fs, long as)
+__device__ __forceinline__ void stress_b_cell(const sw_ssg2d &f, const sw_adj2d &q, const float *tmp, int i, int j)
 {
-    shift_state(f, (long)blockIdx.z * fs);
-    shift_adj(q, (long)blockIdx.z * as);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
-    if (i >= nx || j >= nz) return;
     const long SZ = nx, NC = (long)nx * nz;
     const long p = (long)j * SZ + i;
     const float *c = f.c;
-    const float *T0 = q.tmp, *T1 = q.tmp + NC, *T2 = q.tmp + 2 * NC, *T3 = q.tmp + 3 * NC;
+    const float *T0 = tmp, *T1 = tmp + NC, *T2 = tmp + 2 * NC, *T3 = tmp + 3 * NC;
     float ax = 0.f, az = 0.f;
 #pragma unroll
     for (int id = 0; id < N; ++id) {
@@ -158,16 +149,10 @@ __global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q, lo
 // ----------------------------------------------------------- velocity stage
 // SAVED: planes 0-1 of the saved velocity sweep, as in k_adj_stress_a.
 template <int N, bool SAVED>
-__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, long fs, long as, const float *sv,
-                                                        long ss)
+__device__ __forceinline__ void velocity_a_cell(const sw_ssg2d &f, const sw_adj2d &q, const float *sv, float *tmp,
+                                                int i, int j)
 {
-    shift_state(f, (long)blockIdx.z * fs);
-    shift_adj(q, (long)blockIdx.z * as);
-    if (SAVED) sv += (long)blockIdx.z * ss;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
-    if (i >= nx || j >= nz) return;
     const long SZ = nx, NC = (long)nx * nz;
     const long p = (long)j * SZ + i;
     const float dx = f.dx, dz = f.dz, dt = f.dt;
@@ -227,25 +212,20 @@ __global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, 
         t2 = bx / dx;
         t3 = bz / dz;
     }
-    q.tmp[0 * NC + p] = t0;
-    q.tmp[1 * NC + p] = t1;
-    q.tmp[2 * NC + p] = t2;
-    q.tmp[3 * NC + p] = t3;
+    tmp[0 * NC + p] = t0;
+    tmp[1 * NC + p] = t1;
+    tmp[2 * NC + p] = t2;
+    tmp[3 * NC + p] = t3;
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q, long fs, long as)
+__device__ __forceinline__ void velocity_b_cell(const sw_ssg2d &f, const sw_adj2d &q, const float *tmp, int i, int j)
 {
-    shift_state(f, (long)blockIdx.z * fs);
-    shift_adj(q, (long)blockIdx.z * as);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = f.nx, nz = f.nz;
-    if (i >= nx || j >= nz) return;
     const long SZ = nx, NC = (long)nx * nz;
     const long p = (long)j * SZ + i;
     const float *c = f.c;
-    const float *T0 = q.tmp, *T1 = q.tmp + NC, *T2 = q.tmp + 2 * NC, *T3 = q.tmp + 3 * NC;
+    const float *T0 = tmp, *T1 = tmp + NC, *T2 = tmp + 2 * NC, *T3 = tmp + 3 * NC;
     float axx = 0.f, axz = 0.f, azz = 0.f;
 #pragma unroll
     for (int id = 0; id < N; ++id) {
@@ -262,6 +242,70 @@ __global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q, 
     q.sxz[p] += axz;
     q.szz[p] += azz;
 }
+
+// One thread per cell of shot blockIdx.z: the forward state at + z * fs (SAVED:
+// derivative planes at sv + z * ss), the adjoint block at + z * as.
+#define SW_ADJ_CELL_PROLOGUE                                            \
+    shift_state(f, (long)blockIdx.z * fs);                              \
+    shift_adj(q, (long)blockIdx.z * as);                                \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;                \
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;                \
+    if (i >= f.nx || j >= f.nz) return;
+
+template <int N, bool SAVED>
+__global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, long fs, long as, const float *sv,
+                                                      long ss)
+{
+    SW_ADJ_CELL_PROLOGUE
+    stress_a_cell<N, SAVED>(f, q, SAVED ? sv + (long)blockIdx.z * ss : nullptr, q.tmp, i, j);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_stress_b(sw_ssg2d f, sw_adj2d q, long fs, long as)
+{
+    SW_ADJ_CELL_PROLOGUE
+    stress_b_cell<N>(f, q, q.tmp, i, j);
+}
+
+template <int N, bool SAVED>
+__global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, long fs, long as, const float *sv,
+                                                        long ss)
+{
+    SW_ADJ_CELL_PROLOGUE
+    velocity_a_cell<N, SAVED>(f, q, SAVED ? sv + (long)blockIdx.z * ss : nullptr, q.tmp, i, j);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_velocity_b(sw_ssg2d f, sw_adj2d q, long fs, long as)
+{
+    SW_ADJ_CELL_PROLOGUE
+    velocity_b_cell<N>(f, q, q.tmp, i, j);
+}
+
+// Fused passes of the segment: a gather (pass B) and the next stage's pass A
+// of the same cell, which reads only that cell of what the gather wrote.  The
+// stress stage's pass-A outputs live in q.tmp, the velocity stage's in tv
+// (tv + z * ts for shot z), so neither pass overwrites what the other reads.
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_sb_va(sw_ssg2d f, sw_adj2d q, long as, const float *sv, long ss,
+                                                   float *tv, long ts)
+{
+    const long fs = 0;
+    SW_ADJ_CELL_PROLOGUE
+    stress_b_cell<N>(f, q, q.tmp, i, j);
+    velocity_a_cell<N, true>(f, q, sv + (long)blockIdx.z * ss, tv + (long)blockIdx.z * ts, i, j);
+}
+
+template <int N, bool WITH_SA>
+__global__ void __launch_bounds__(256) k_adj_vb_sa(sw_ssg2d f, sw_adj2d q, long as, const float *sv, long ss,
+                                                   float *tv, long ts)
+{
+    const long fs = 0;
+    SW_ADJ_CELL_PROLOGUE
+    velocity_b_cell<N>(f, q, tv + (long)blockIdx.z * ts, i, j);
+    if (WITH_SA) stress_a_cell<N, true>(f, q, sv + (long)blockIdx.z * ss, q.tmp, i, j);
+}
+#undef SW_ADJ_CELL_PROLOGUE
 
 // Residual injection: adjoint of the receiver gather data[it] = field[idx].
 // One thread per (component, receiver).  Receivers at the same cell add up;
@@ -366,6 +410,28 @@ int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch
         hipLaunchKernelGGL((k_adj_velocity_b<N>), grd, blk, 0, st, *f, *q, fs, as);
     });
     return launch_status("2-D adjoint velocity stage");
+}
+
+// Passes of the fused segment schedule (SAVED derivative planes sv + z * ss,
+// velocity-stage scratch tv + z * ts): 0 stress pass A alone, 1 stress
+// gather + velocity pass A, 2 velocity gather + stress pass A, 3 velocity
+// gather alone.
+int launch_adj2d_fused(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, int pass, const float *sv, long ss,
+                       float *tv, long ts, hipStream_t st)
+{
+    const dim3 blk(64, 4, 1);
+    SW_DISPATCH_ORDER(f->order, {
+        const dim3 grd = grid2(*f, blk, nbatch);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_adj_stress_a<N, true>), grd, blk, 0, st, *f, *q, 0L, as, sv, ss);
+        else if (pass == 1)
+            hipLaunchKernelGGL((k_adj_sb_va<N>), grd, blk, 0, st, *f, *q, as, sv, ss, tv, ts);
+        else if (pass == 2)
+            hipLaunchKernelGGL((k_adj_vb_sa<N, true>), grd, blk, 0, st, *f, *q, as, sv, ss, tv, ts);
+        else
+            hipLaunchKernelGGL((k_adj_vb_sa<N, false>), grd, blk, 0, st, *f, *q, as, sv, ss, tv, ts);
+    });
+    return launch_status("2-D adjoint fused pass");
 }
 
 int launch_adj2d_stress(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st)

@@ -46,6 +46,8 @@ int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch
                                 const float *sv, long ss, hipStream_t st);
 int launch_ssg2d_sweeps_saved(const sw_ssg2d *g, int nbatch, long bs, bool stress, float *sv, long ss,
                               const Fold2D &fo, hipStream_t st);
+int launch_adj2d_fused(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, int pass, const float *sv, long ss,
+                       float *tv, long ts, hipStream_t st);
 int launch_adj2d_inject_batch(const sw_adj2d *q, const sw_receivers *r, int it, int nbatch, long as, long ds,
                               hipStream_t st);
 int launch_adj2d_model_gradient_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, const float *vp,
@@ -761,14 +763,41 @@ int sw_ssg2d_adjoint_segment_batch(const sw_ssg2d *g, int32_t nbatch, int64_t st
             hipLaunchKernelGGL(k_inject2d_batch, dim3(nbatch), dim3(1), 0, st, *g, *src, sc, it, bs);
         if ((rc = launch_ssg2d_sweeps_saved(g, nbatch, bs, true, sv, save_n, fo, st))) return rc;
     }
+    // Backwards through the steps.  With velocity residuals only, each stage's
+    // gather runs fused with the next stage's per-cell pass (two launches per
+    // step; the residuals enter q.vx, q.vz, which the fused stress pass does
+    // not read); stress residuals take the four separate passes.
+    bool fuse = true;
+    for (int c = 0; c < residual->ncomp; ++c) fuse = fuse && (residual->comp[c] == 0 || residual->comp[c] == 2);
+    float *tv = work + (size_t)n * step_n;  // velocity-stage pass-A outputs, 4 planes per shot
+    const long tvs = 4 * P;
+    auto inject = [&](int t) {
+        return residual->nrec ? launch_adj2d_inject_batch(adj, residual, it0 + t, nbatch, as, data_stride, st) : SW_OK;
+    };
     for (int t = n - 1; t >= 0; --t) {
         const float *sv = work + (size_t)t * step_n;
-        if (residual->nrec && (rc = launch_adj2d_inject_batch(adj, residual, it0 + t, nbatch, as, data_stride, st)))
-            return rc;
-        if ((rc = launch_adj2d_stress_batch(g, adj, nbatch, 0, as, sv, save_n, st))) return rc;
+        if (!fuse) {
+            if ((rc = inject(t))) return rc;
+            if ((rc = launch_adj2d_stress_batch(g, adj, nbatch, 0, as, sv, save_n, st))) return rc;
+            if (force_source(src))
+                hipLaunchKernelGGL(k_adj_force_batch, dim3(nbatch), dim3(1), 0, st, *adj, *src, sc, it0 + t, as,
+                                   g->nx);
+            if ((rc = launch_adj2d_velocity_batch(g, adj, nbatch, 0, as, sv, save_n, st))) return rc;
+            continue;
+        }
+        if (t == n - 1) {
+            if ((rc = inject(t))) return rc;
+            if ((rc = launch_adj2d_fused(g, adj, nbatch, as, 0, sv, save_n, tv, tvs, st))) return rc;
+        }
+        if ((rc = launch_adj2d_fused(g, adj, nbatch, as, 1, sv, save_n, tv, tvs, st))) return rc;
         if (force_source(src))
             hipLaunchKernelGGL(k_adj_force_batch, dim3(nbatch), dim3(1), 0, st, *adj, *src, sc, it0 + t, as, g->nx);
-        if ((rc = launch_adj2d_velocity_batch(g, adj, nbatch, 0, as, sv, save_n, st))) return rc;
+        if (t > 0) {
+            if ((rc = inject(t - 1))) return rc;
+            if ((rc = launch_adj2d_fused(g, adj, nbatch, as, 2, sv - step_n, save_n, tv, tvs, st))) return rc;
+        } else if ((rc = launch_adj2d_fused(g, adj, nbatch, as, 3, sv, save_n, tv, tvs, st))) {
+            return rc;
+        }
     }
     return launch_status("ssg2d adjoint segment batch");
 }

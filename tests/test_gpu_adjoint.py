@@ -112,12 +112,15 @@ def test_chain_transpose_dot_product():
     assert abs(lhs - rhs) <= 1e-4 * max(abs(lhs), abs(rhs), 1.0), (lhs, rhs)
 
 
-@pytest.mark.parametrize("source", ["explosive", "force_vz"])
-def test_segment_matches_stage_by_stage_schedule(source):
+@pytest.mark.parametrize("source,components", [("explosive", ("vx", "vz")), ("force_vz", ("vx", "vz")),
+                                               ("explosive", ("vz", "sxx"))])
+def test_segment_matches_stage_by_stage_schedule(source, components):
     """sw_ssg2d_adjoint_segment (forward derivative sums saved by the
-    recomputed sweeps) leaves the adjoint state and the five coefficient
-    accumulators bit-equal to the per-stage calls on full forward states kept
-    before and after every velocity sweep (which recompute those sums)."""
+    recomputed sweeps; with velocity residuals each gather fused with the next
+    stage's per-cell pass, with a stress residual the four passes apart)
+    leaves the adjoint state and the five coefficient accumulators bit-equal
+    to the per-stage calls on full forward states kept before and after every
+    velocity sweep (which recompute those sums)."""
     from seiswave_taichi_amd import _lib as L
     from seiswave_taichi_amd.adjoint import Elastic2DModeling
     d, nt, K = (10.0, 10.0), 60, 25
@@ -127,7 +130,7 @@ def test_segment_matches_stage_by_stage_schedule(source):
     p = pml_dict(4700.0, 8, d, 20.0, 2700.0, dims=2)
     wavelet = 1e8 * ricker2(np.arange(nt), 8e-4, 1.5 / 20.0, 20.0, 1.0)
     m = Elastic2DModeling(shape, *d, 8e-4, nt, 2, p, [shot], rec, freq=20.0, checkpoint=K, wavelet=wavelet,
-                          source=source, batch=False)
+                          source=source, batch=False, components=components)
     vs, vp, rho = layered_model(shape, seed=6)
     m.set_model(torch.from_numpy(vp), torch.from_numpy(vs), torch.from_numpy(rho))
     lib, st = m.lib, m.stream()
