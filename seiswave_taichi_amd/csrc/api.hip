@@ -75,13 +75,13 @@ int check_ssg3d(const sw_ssg3d *g, bool spml)
     SW_REQUIRE(g->nx >= need && g->ny >= need && g->nz >= need,
                "grid %dx%dx%d too small for order %d (each axis needs >= %d)", g->nx, g->ny, g->nz,
                g->order, need);
-    // The sweeps address a z chunk (at most 64 planes) and its stencil halo
+    // The sweeps address a z chunk (kZChunk planes) and its stencil halo
     // with 32-bit byte offsets from pointers rebased per block, the z memory
     // slab with 32-bit offsets from its base, and receivers by int32 cells.
     const long cells = (long)g->nx * g->ny * g->nz, plane = 4L * g->nx * g->ny;
     SW_REQUIRE(cells <= INT32_MAX, "grid %dx%dx%d: %ld cells, above the 2^31-1 an int32 cell index reaches",
                g->nx, g->ny, g->nz, cells);
-    SW_REQUIRE((64L + 2 * g->order + 1) * plane <= (1L << 32),
+    SW_REQUIRE(((long)kZChunk + 2 * g->order + 1) * plane <= (1L << 32),
                "planes of %dx%d cells too large: a z chunk of the sweeps must stay below 4 GiB", g->nx, g->ny);
     // the six memory variables of each axis share one 32-bit offset range
     const long wx = g->px.lo + g->nx - g->px.hi, wy = g->py.lo + g->ny - g->py.hi, wz = g->pz.lo + g->nz - g->pz.hi;

@@ -51,7 +51,7 @@ namespace {
 // A launch covers one box of cells: columns [i0, i0+w) x [j0, j0+h) over
 // planes [k0, k1).  The box is cut into blocks of kThreads consecutive
 // columns (x fastest, so no lane idles on a ragged x edge and a wave reads at
-// most two row segments per tap) times chunks of cz planes.
+// most two row segments per tap) times chunks of kZChunk planes.
 constexpr int kThreads = 256;
 
 struct Box {
@@ -60,7 +60,7 @@ struct Box {
 
 struct Geo {
     Box box;
-    int cz, nblock;
+    int nblock;
 };
 
 struct Cell {
@@ -80,8 +80,8 @@ __device__ __forceinline__ Cell box_cell(const Geo &geo)
         c.i = B.i0 + t % B.w;
         c.j = B.j0 + t / B.w;
     }
-    c.k0 = B.k0 + zb * geo.cz;
-    c.k1 = min(c.k0 + geo.cz, B.k1);
+    c.k0 = B.k0 + zb * kZChunk;
+    c.k1 = min(c.k0 + kZChunk, B.k1);
     return c;
 }
 
@@ -582,9 +582,6 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
         stress_src3d(g, fo.stress, fo.mt, fo.amp[fo.it_next], ((long)fo.iz * ny + j) * nx + i);
 }
 
-#ifndef SW_CZ0
-#define SW_CZ0 8
-#endif
 
 // Cells a sweep updates: [N, n-N) on every axis, in one launch (PM = 7) or
 // in five launches whose boxes all keep full-width x rows (long coalesced
@@ -601,10 +598,14 @@ struct Plan {
     Geo g[5];
 };
 
-// Chunk length: SW_CZ0 planes, doubled (up to 64) while the launch keeps at
-// least 32 blocks per CU, so that the last round of blocks is a small part of
-// a launch and the window warm-up (2N-1 planes per chunk) a small part of
-// large grids.
+// Chunks of kZChunk planes on every grid.  Short chunks re-read the 2N-1
+// window planes of each chunk, but those are the planes the block of the
+// chunk below reads at about the same time on the same XCD (consecutive
+// logical blocks share an XCD), so they come from L2; what the many short
+// blocks gain is an even spread over the CUs and a short last round.
+// Measured on MI355X (ms per step, chunk length 2 / 3 / 4 / 8 / 16): 200^3
+// 0.309 / 0.310 / 0.307 / 0.318 / -; 600x600x400 - / - / 4.66 / 4.74 / 5.08
+// (5.03 with 64-plane chunks).
 void add_box(Plan &p, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
 {
     if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
@@ -615,11 +616,7 @@ void add_box(Plan &p, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
     b.j0 = j0; b.h = j1 - j0;
     b.k0 = k0; b.k1 = k1;
     b.ncb = (b.w * b.h + kThreads - 1) / kThreads;
-    auto blocks = [&](int cz) { return (long)b.ncb * ((k1 - k0 + cz - 1) / cz); };
-    int cz = SW_CZ0;
-    while (cz < 64 && blocks(2 * cz) >= 32L * 256) cz *= 2;
-    geo.cz = cz;
-    geo.nblock = (int)blocks(cz);
+    geo.nblock = (int)((long)b.ncb * ((k1 - k0 + kZChunk - 1) / kZChunk));
 }
 
 Plan make_plan(const sw_ssg3d &g, int N)

@@ -12,6 +12,10 @@
 
 namespace sw {
 
+// Planes per z chunk of the 3-D sweeps: a block marches its columns over one
+// chunk, and its 32-bit offsets span the chunk plus the stencil halo.
+constexpr int kZChunk = 4;
+
 // Per-thread message of the last failure (read back by sw_last_error).
 void set_error(const char *fmt, ...);
 

@@ -318,7 +318,7 @@ def test_grid_size_limits_are_checked():
 
     rc, msg = run(1300, 1300, 1300)  # 2.2e9 cells
     assert rc == -1 and "int32 cell index" in msg
-    rc, msg = run(4000, 4000, 40)  # 16e6-cell planes: a 69-plane chunk exceeds 4 GiB
+    rc, msg = run(11000, 11000, 8)  # 121e6-cell planes: a chunk of 4 planes + halo exceeds 4 GiB
     assert rc == -1 and "z chunk" in msg
     rc, msg = run(200, 3000, 3000, lo=400)  # 800-cell x slabs: 24*800*3000*3000 B
     assert rc == -1 and "memory slabs" in msg
