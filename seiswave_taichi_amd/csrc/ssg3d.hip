@@ -54,6 +54,16 @@ namespace {
 // most two row segments per tap) times chunks of kZChunk planes.
 constexpr int kThreads = 256;
 
+// Blocks run in groups of kGroup column blocks (8,192 columns) that march
+// through all z chunks before the next group starts.  With 4-plane chunks a
+// chunk's window planes are the ones the same group read one chunk earlier,
+// still in the XCD's L2, while a group is wide enough that its y halo rows
+// are a small part of its reads.  Measured on MI355X, ms per step with groups
+// of 4 / 8 / 16 / 24 / 32 / 48 / 64 / 96 column blocks: 200^3 0.327 / 0.322 /
+// 0.322 / 0.313 / 0.305 / 0.308 / 0.305 / 0.307, against 0.318 in
+// plane-by-plane order; 600x600x400 4.77 (32), 4.73 (64), 4.91 plane by plane.
+constexpr int kGroup = 32;
+
 struct Box {
     int i0, j0, w, h, k0, k1, ncb;  // ncb: column blocks of the box
 };
@@ -71,7 +81,13 @@ __device__ __forceinline__ Cell box_cell(const Geo &geo)
 {
     const int b = xcd_logical_block(blockIdx.x, geo.nblock);
     const Box &B = geo.box;
-    const int cb = b % B.ncb, zb = b / B.ncb;
+    // logical order: groups of kGroup column blocks, each group's z chunks
+    // bottom to top, its column blocks fastest within a chunk (the last
+    // group may be narrower)
+    const int nzc = (B.k1 - B.k0 + kZChunk - 1) / kZChunk;
+    const int gi = b / (kGroup * nzc), r = b - gi * kGroup * nzc;
+    const int gw = min(kGroup, B.ncb - gi * kGroup);
+    const int zb = r / gw, cb = gi * kGroup + r - zb * gw;
     const int t = cb * kThreads + (int)threadIdx.x;
     Cell c;
     if (t >= B.w * B.h) {
