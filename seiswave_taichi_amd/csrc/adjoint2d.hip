@@ -37,7 +37,7 @@ __device__ __forceinline__ float fwd_cpml(float D, float m, float a, float b, fl
 
 // ------------------------------------------------------------- stress stage
 // SAVED: the forward derivatives come from the planes the saved stress sweep
-// wrote (sv + blockIdx.z * ss, planes 2-4) instead of being recomputed from
+// wrote (sv + z * ss for shot z, planes 2-4) instead of being recomputed from
 // the forward state; they are the same values, so the result is the same.
 // The four stage passes of one cell (i, j) of one shot (f, q, sv already at
 // the shot's offsets), writing / reading the pass-A outputs at tmp (4 planes).
@@ -243,13 +243,17 @@ __device__ __forceinline__ void velocity_b_cell(const sw_ssg2d &f, const sw_adj2
     q.szz[p] += azz;
 }
 
-// One thread per cell of shot blockIdx.z: the forward state at + z * fs (SAVED:
-// derivative planes at sv + z * ss), the adjoint block at + z * as.
+// One thread per cell of shot bk.z: the forward state at + z * fs (SAVED:
+// derivative planes at sv + z * ss), the adjoint block at + z * as.  The
+// stage kernels keep the dispatch order of their blocks: with the XCD
+// renumbering the sweeps use, the two fused kernels of a backward step ran
+// 70.1 and 84.0 us on MI355X (16 shots of 800x300) against 65.9 and 78.8.
 #define SW_ADJ_CELL_PROLOGUE                                            \
-    shift_state(f, (long)blockIdx.z * fs);                              \
-    shift_adj(q, (long)blockIdx.z * as);                                \
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;                \
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;                \
+    const Blk bk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};    \
+    shift_state(f, (long)bk.z * fs);                                    \
+    shift_adj(q, (long)bk.z * as);                                      \
+    const int i = bk.x * blockDim.x + threadIdx.x;                      \
+    const int j = bk.y * blockDim.y + threadIdx.y;                      \
     if (i >= f.nx || j >= f.nz) return;
 
 template <int N, bool SAVED>
@@ -257,7 +261,7 @@ __global__ void __launch_bounds__(256) k_adj_stress_a(sw_ssg2d f, sw_adj2d q, lo
                                                       long ss)
 {
     SW_ADJ_CELL_PROLOGUE
-    stress_a_cell<N, SAVED>(f, q, SAVED ? sv + (long)blockIdx.z * ss : nullptr, q.tmp, i, j);
+    stress_a_cell<N, SAVED>(f, q, SAVED ? sv + (long)bk.z * ss : nullptr, q.tmp, i, j);
 }
 
 template <int N>
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(256) k_adj_velocity_a(sw_ssg2d f, sw_adj2d q, 
                                                         long ss)
 {
     SW_ADJ_CELL_PROLOGUE
-    velocity_a_cell<N, SAVED>(f, q, SAVED ? sv + (long)blockIdx.z * ss : nullptr, q.tmp, i, j);
+    velocity_a_cell<N, SAVED>(f, q, SAVED ? sv + (long)bk.z * ss : nullptr, q.tmp, i, j);
 }
 
 template <int N>
@@ -293,7 +297,7 @@ __global__ void __launch_bounds__(256) k_adj_sb_va(sw_ssg2d f, sw_adj2d q, long 
     const long fs = 0;
     SW_ADJ_CELL_PROLOGUE
     stress_b_cell<N>(f, q, q.tmp, i, j);
-    velocity_a_cell<N, true>(f, q, sv + (long)blockIdx.z * ss, tv + (long)blockIdx.z * ts, i, j);
+    velocity_a_cell<N, true>(f, q, sv + (long)bk.z * ss, tv + (long)bk.z * ts, i, j);
 }
 
 template <int N, bool WITH_SA>
@@ -302,8 +306,8 @@ __global__ void __launch_bounds__(256) k_adj_vb_sa(sw_ssg2d f, sw_adj2d q, long 
 {
     const long fs = 0;
     SW_ADJ_CELL_PROLOGUE
-    velocity_b_cell<N>(f, q, tv + (long)blockIdx.z * ts, i, j);
-    if (WITH_SA) stress_a_cell<N, true>(f, q, sv + (long)blockIdx.z * ss, q.tmp, i, j);
+    velocity_b_cell<N>(f, q, tv + (long)bk.z * ts, i, j);
+    if (WITH_SA) stress_a_cell<N, true>(f, q, sv + (long)bk.z * ss, q.tmp, i, j);
 }
 #undef SW_ADJ_CELL_PROLOGUE
 
@@ -385,7 +389,7 @@ inline dim3 grid2(const sw_ssg2d &g, dim3 blk, int nbatch = 1)
 int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as, const float *sv,
                               long ss, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     SW_DISPATCH_ORDER(f->order, {
         const dim3 grd = grid2(*f, blk, nbatch);
         if (sv)
@@ -400,7 +404,7 @@ int launch_adj2d_stress_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, 
 int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long fs, long as,
                                 const float *sv, long ss, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     SW_DISPATCH_ORDER(f->order, {
         const dim3 grd = grid2(*f, blk, nbatch);
         if (sv)
@@ -419,7 +423,7 @@ int launch_adj2d_velocity_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch
 int launch_adj2d_fused(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, int pass, const float *sv, long ss,
                        float *tv, long ts, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     SW_DISPATCH_ORDER(f->order, {
         const dim3 grd = grid2(*f, blk, nbatch);
         if (pass == 0)
@@ -460,7 +464,7 @@ int launch_adj2d_inject(const sw_adj2d *q, const sw_receivers *r, int it, hipStr
 int launch_adj2d_model_gradient_batch(const sw_ssg2d *f, const sw_adj2d *q, int nbatch, long as, const float *vp,
                                       const float *vs, float *gvp, float *gvs, float *grho, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     hipLaunchKernelGGL(k_model_gradient, grid2(*f, blk), blk, 0, st, *f, *q, vp, vs, gvp, gvs, grho, nbatch, as);
     return launch_status("2-D model gradient");
 }

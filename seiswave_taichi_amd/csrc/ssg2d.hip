@@ -58,10 +58,11 @@ struct Rot {
 template <int N, int SCHEME, bool SAVE>
 __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs, float *sv, long ss, Fold2D fo)
 {
-    shift_state(g, (long)blockIdx.z * bs);
-    if (SAVE) sv += (long)blockIdx.z * ss;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
+    const Blk bk = xcd_block3();
+    shift_state(g, (long)bk.z * bs);
+    if (SAVE) sv += (long)bk.z * ss;
+    const int i = bk.x * blockDim.x + threadIdx.x + N;
+    const int j = bk.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
     if (i >= nx - N || j >= nz - N) return;
     const long SZ = nx, NC = (long)nx * nz;
@@ -142,26 +143,27 @@ __global__ void __launch_bounds__(256) k_velocity2d(sw_ssg2d g, long bs, float *
             g.vz[p] += (dszzdz + dsxzdx) * dt / rho;
         }
     }
-    if (SCHEME != SW_2D_RSG && fo.force && i == fo.ix[blockIdx.z] && j == fo.iz[blockIdx.z])
+    if (SCHEME != SW_2D_RSG && fo.force && i == fo.ix[bk.z] && j == fo.iz[bk.z])
         force_src2d(g, fo.force, fo.amp[fo.it_force], p);
 }
 
 template <int N, int SCHEME, bool SAVE>
 __global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs, float *sv, long ss, Fold2D fo)
 {
-    shift_state(g, (long)blockIdx.z * bs);
-    if (SAVE) sv += (long)blockIdx.z * ss;
-    if (fo.record && blockIdx.x == 0 && blockIdx.y == 0) {
+    const Blk bk = xcd_block3();
+    shift_state(g, (long)bk.z * bs);
+    if (SAVE) sv += (long)bk.z * ss;
+    if (fo.record && bk.x == 0 && bk.y == 0) {
         // velocity receivers of this step: data[(it*ncomp + comp)*nrec + r]
         const int n = fo.rec.nrec * fo.rec.ncomp;
-        float *out = fo.rec.data + (long)blockIdx.z * fo.ds + (long)fo.it_rec * n;
+        float *out = fo.rec.data + (long)bk.z * fo.ds + (long)fo.it_rec * n;
         for (int t = threadIdx.y * blockDim.x + threadIdx.x; t < n; t += blockDim.x * blockDim.y) {
             const int comp = t / fo.rec.nrec, r = t - comp * fo.rec.nrec;
             out[t] = (fo.rec.comp[comp] == 0 ? g.vx : g.vz)[fo.rec.idx[r]];
         }
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + N;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + N;
+    const int i = bk.x * blockDim.x + threadIdx.x + N;
+    const int j = bk.y * blockDim.y + threadIdx.y + N;
     const int nx = g.nx, nz = g.nz;
     if (i >= nx - N || j >= nz - N) return;
     const long SZ = nx, NC = (long)nx * nz;
@@ -251,7 +253,7 @@ __global__ void __launch_bounds__(256) k_stress2d(sw_ssg2d g, long bs, float *sv
             g.sxz[p] += (mu * (dvxdz + dvzdx)) * dt;
         }
     }
-    if (SCHEME != SW_2D_RSG && fo.stress && i == fo.ix[blockIdx.z] && j == fo.iz[blockIdx.z])
+    if (SCHEME != SW_2D_RSG && fo.stress && i == fo.ix[bk.z] && j == fo.iz[bk.z])
         stress_src2d(g, fo.stress, fo.m0, fo.m2, fo.m4, fo.amp[fo.it_next], p);
 }
 
@@ -264,7 +266,7 @@ inline dim3 box_grid2(const sw_ssg2d &g, int N, dim3 blk, int nbatch)
 
 int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
         if (g->scheme == SW_2D_VS)
@@ -279,7 +281,7 @@ int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, const Fo
 
 int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);
         if (g->scheme == SW_2D_VS)
@@ -295,7 +297,7 @@ int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold
 int launch_ssg2d_sweeps_saved(const sw_ssg2d *g, int nbatch, long bs, bool stress, float *sv, long ss,
                               const Fold2D &fo, hipStream_t st)
 {
-    const dim3 blk(64, 4, 1);
+    const dim3 blk(kTile2DX, kTile2DY, 1);
     SW_REQUIRE(g->scheme == SW_2D_VS, "saved sweeps exist for the VS scheme only");
     SW_DISPATCH_ORDER(g->order, {
         const dim3 grd = box_grid2(*g, N, blk, nbatch);

@@ -10,15 +10,6 @@ namespace sw {
 
 namespace {
 
-constexpr int kXCD = 8;  // XCDs per MI355X; any value keeps the renumbering a bijection
-
-__device__ __forceinline__ int xcd_logical_block(int b, int nb)
-{
-    const int q = nb / kXCD, r = nb % kXCD;
-    const int x = b % kXCD, s = b / kXCD;
-    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
-}
-
 // Fields are addressed by 32-bit byte offsets from their base pointers: one
 // offset register per column serves every field, and the wave-uniform tap
 // deltas fold into the load instructions.  A block rebases the full-grid
