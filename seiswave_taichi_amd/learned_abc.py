@@ -146,6 +146,19 @@ def courant(vp_max, d, dt, order=2):
 # CPML profile of every learned-boundary run, independent of the source: the
 # networks learn the error of one particular thin layer.
 PML_FREQ = 15.0
+# Target reflection coefficient of the CPML profile (d0 = -3 vp ln R / 2L):
+# the wide layers keep the 1e-6 of the reference's drivers; a layer of at most
+# THIN_MAX cells uses THIN_RCOEF, since a profile that steep over 4 cells
+# reflects from its own discretisation.  Reflected energy of a 4-cell layer
+# relative to the 20-cell one (homogeneous 3.5 km/s model, 15 Hz moment
+# source, 200 steps, CPU oracle): R = 1e-6: 8.3e4, 1e-4: 7.7e3, 1e-3: 3.2e3,
+# 1e-2: 1.7e4, 1e-1: 7.0e5; kmax 2 or 4, alpha_max 0 or 30 pi at R = 1e-3
+# were all worse.
+WIDE_RCOEF, THIN_RCOEF, THIN_MAX = 1e-6, 1e-3, 8
+
+
+def pml_rcoef(npml):
+    return THIN_RCOEF if npml <= THIN_MAX else WIDE_RCOEF
 
 
 def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
@@ -159,7 +172,7 @@ def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
                     np.zeros((1, 1), np.int64) + cell[1], np.zeros((1, 1), np.int64) + cell[2], mt3, 1.0, nt, 2, f,
                     source={"kind": "moment", "cell": cell, "mt": mt, "wavelet": amps})
     p = {"vp_max": float(vp.max()), "pml_x_thick": npml * d, "pml_y_thick": npml * d, "pml_z_thick": npml * d,
-         "Rcoef": 1e-6, "theta": 1, "alpha_max_pml": np.pi * PML_FREQ, "kmax_pml": 1.0}
+         "Rcoef": pml_rcoef(npml), "theta": 1, "alpha_max_pml": np.pi * PML_FREQ, "kmax_pml": 1.0}
     w.SetADEPML3D([True] * 6, p)
     return w
 
