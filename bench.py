@@ -10,7 +10,8 @@ tensor (10 Hz Ricker) at the centre, 50x50 receivers at k = s+5; 1500 steps.
 
 Prints one JSON line: Gcells/s (nx*ny*nz*steps/t), the roofline of the
 dominant kernel (algorithmic HBM bytes per launch / measured launch time
-against 8 TB/s) and a CPU baseline (the oracle's C restatement with OpenMP on
+against 8 TB/s, and against the measured read + write rate of a 1 GiB
+device copy, the practical ceiling of a streaming kernel) and a CPU baseline (the oracle's C restatement with OpenMP on
 this host's cores, on a bounded sample of the same workload).
 """
 from __future__ import annotations
@@ -112,6 +113,26 @@ def traffic_from_profile(kernel_tag, workload_name):
         return None
     with open(p) as fh:
         return json.load(fh).get("bytes_per_launch")
+
+
+def copy_ceiling_gbs(gib=1.0, reps=5):
+    """Read + write rate of a device-to-device copy of `gib` GiB (bytes moved /
+    time, HIP events on the copy's stream): the rate a pure streaming kernel
+    reaches on this GPU, as a practical ceiling beside the 8 TB/s peak."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    t = a.elapsed_time(b) / 1e3 / reps
+    del src, dst
+    return 2.0 * n * 4 / t / 1e9
 
 
 def cpu_baseline(w, budget_s=15.0):
@@ -388,6 +409,9 @@ def main():
         out["roofline_all_split_components"] = {
             "split_cells": int(n_split), "algorithmic_bytes_per_launch": int(b_all[dom]),
             "achieved": round(a_all, 1), "frac": round(a_all / HBM_PEAK_GBS, 4)}
+    ceiling = copy_ceiling_gbs()
+    out["roofline"]["copy_ceiling_GBs"] = round(ceiling, 1)
+    out["roofline"]["frac_of_copy_ceiling"] = round(ach / ceiling, 4)
     if args.copy_calibration > 0:
         n = int(args.copy_calibration * (1 << 30)) // 4
         src_buf = torch.zeros(n, dtype=torch.float32, device="cuda")
