@@ -614,14 +614,15 @@ struct Plan {
     Geo g[5];
 };
 
-// Chunks of kZChunk planes on every grid.  Short chunks re-read the 2N-1
-// window planes of each chunk, but those are the planes the block of the
-// chunk below reads at about the same time on the same XCD (consecutive
-// logical blocks share an XCD), so they come from L2; what the many short
-// blocks gain is an even spread over the CUs and a short last round.
-// Measured on MI355X (ms per step, chunk length 2 / 3 / 4 / 8 / 16): 200^3
-// 0.309 / 0.310 / 0.307 / 0.318 / -; 600x600x400 - / - / 4.66 / 4.74 / 5.08
-// (5.03 with 64-plane chunks).
+// Chunks of kZChunk planes on every grid: the many short blocks spread
+// evenly over the CUs and keep the last round of a launch short, and the
+// 2N-1 window planes a chunk re-reads are those its group of column blocks
+// read one chunk earlier (kGroup), still in L2.  Measured on MI355X, ms per
+// step: in plane-by-plane block order with chunks of 2 / 3 / 4 / 8 / 16
+// planes, 200^3 0.309 / 0.310 / 0.307 / 0.318 / -, 600x600x400 - / - / 4.66 /
+// 4.74 / 5.08 (5.03 with 64-plane chunks); in groups of 32 column blocks
+// with chunks of 2 / 4 / 6 / 8 planes, 200^3 0.309 / 0.305 / 0.313 / 0.317,
+// 600x600x400 4.77 (4) / 4.92 (8).
 void add_box(Plan &p, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
 {
     if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
