@@ -38,9 +38,9 @@ constexpr int kTile2DX = 64, kTile2DY = 4;
 // Block coordinates of a launch grid of 3 dimensions with the XCD
 // renumbering applied to its x-fastest linear order (a 2-D tile grid per
 // shot, shots along z): the tiles above and below a tile, whose rows its
-// z stencil reads, run on the same XCD.  On MI355X the 16-shot 800x300
-// gradient took 0.816 s against 0.888 s in dispatch order (forward sweeps
-// 59 against 46 Gcells/s), a 400^2 step 10.0 against 11.6 us.
+// z stencil reads, run on the same XCD.  On MI355X the forward sweeps of 16
+// shots of 800x300 ran 59 against 46 Gcells/s in dispatch order, a 400^2
+// step 10.0 against 11.6 us.
 struct Blk {
     int x, y, z;
 };
