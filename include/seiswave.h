@@ -134,6 +134,13 @@ int sw_ssg3d_stress(const sw_ssg3d *g, void *stream);
 /* Split-field velocity / stress loops: src/Elastic3D_SSG_SPML.py:182-259, :296-391. */
 int sw_ssg3d_velocity_spml(const sw_ssg3d *g, void *stream);
 int sw_ssg3d_stress_spml(const sw_ssg3d *g, void *stream);
+/* Launch plan of the two 3-D sweeps for this descriptor (no counterpart in
+ * the reference, whose loops are one Taichi task each): returns the number of
+ * launches per sweep (1, or 5 when the CPML-free bulk is >= 75 % of the
+ * updated cells) and writes up to max CPML axis masks (bit 0 x, 1 y, 2 z:
+ * the axes whose CPML or split-field code the launch instantiates).  Host
+ * only: no device access. */
+int sw_ssg3d_launch_plan(const sw_ssg3d *g, int32_t *pm, int32_t max);
 /* Source injection for step it (the serial statements of the reference kernels). */
 int sw_ssg3d_inject(const sw_ssg3d *g, const sw_source *src, int32_t it, void *stream);
 /* Receiver gather for step it (src/Elastic3D_SSG_CPML.py:309, Tensor :344-345). */

@@ -81,6 +81,7 @@ SIGNATURES = {
     "sw_ssg3d_stress": [_P3, C.c_void_p],
     "sw_ssg3d_velocity_spml": [_P3, C.c_void_p],
     "sw_ssg3d_stress_spml": [_P3, C.c_void_p],
+    "sw_ssg3d_launch_plan": [_P3, C.POINTER(C.c_int32), C.c_int32],
     "sw_ssg3d_inject": [_P3, _PS, C.c_int32, C.c_void_p],
     "sw_ssg3d_record": [_P3, _PR, C.c_int32, C.c_void_p],
     "sw_ssg3d_run": [_P3, _PS, _PR, C.c_int32, C.c_int32, C.c_void_p],
@@ -171,3 +172,14 @@ def u8ptr(t):
 
 def i32ptr(t):
     return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_int32))
+
+
+def launch_plan(g) -> list:
+    """CPML axis masks of the launches each 3-D sweep makes for descriptor
+    ``g`` (one entry per launch; bit 0 x, 1 y, 2 z).  Host only."""
+    lib = load()
+    pm = (C.c_int32 * 8)()
+    n = lib.sw_ssg3d_launch_plan(C.byref(g), pm, 8)
+    if n < 0:
+        check(n, "sw_ssg3d_launch_plan")
+    return list(pm[:n])

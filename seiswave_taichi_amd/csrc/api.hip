@@ -23,6 +23,7 @@ int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st);
+int ssg3d_plan_masks(const sw_ssg3d *g, int32_t *pm, int32_t max);
 int launch_ssg3d_velocity_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
 int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st);
@@ -67,9 +68,14 @@ static int check_axis(const sw_axis &a, int n, const char *name, bool need_cpml,
     return SW_OK;
 }
 
-int check_ssg3d(const sw_ssg3d *g, bool spml)
+int check_ssg3d(const sw_ssg3d *g)
 {
     SW_REQUIRE(g != nullptr, "null descriptor");
+    SW_REQUIRE(g->cls == SW_CLASS_CPML || g->cls == SW_CLASS_TENSOR || g->cls == SW_CLASS_SPML,
+               "unknown 3-D class %d", g->cls);
+    // the split-field class keeps split arrays and damping profiles, the
+    // others CPML memory slabs and coefficients
+    const bool spml = g->cls == SW_CLASS_SPML;
     SW_REQUIRE(g->order >= 1 && g->order <= SW_MAX_ORDER, "order %d outside 1..%d", g->order, SW_MAX_ORDER);
     const int need = 2 * g->order + 2;
     SW_REQUIRE(g->nx >= need && g->ny >= need && g->nz >= need,
@@ -416,31 +422,42 @@ int sw_device_init(int device)
 
 int sw_ssg3d_velocity(const sw_ssg3d *g, void *stream)
 {
-    int rc = check_ssg3d(g, false);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
-    SW_REQUIRE(g->cls == SW_CLASS_CPML || g->cls == SW_CLASS_TENSOR, "velocity sweep needs a CPML/Tensor descriptor");
+    SW_REQUIRE(g->cls != SW_CLASS_SPML, "velocity sweep needs a CPML/Tensor descriptor (split-field: sw_ssg3d_velocity_spml)");
     return launch_ssg3d_velocity(g, as_stream(stream));
 }
 
 int sw_ssg3d_stress(const sw_ssg3d *g, void *stream)
 {
-    int rc = check_ssg3d(g, false);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
+    SW_REQUIRE(g->cls != SW_CLASS_SPML, "stress sweep needs a CPML/Tensor descriptor (split-field: sw_ssg3d_stress_spml)");
     return launch_ssg3d_stress(g, as_stream(stream));
 }
 
 int sw_ssg3d_velocity_spml(const sw_ssg3d *g, void *stream)
 {
-    int rc = check_ssg3d(g, true);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
+    SW_REQUIRE(g->cls == SW_CLASS_SPML, "split-field velocity sweep needs a split-field descriptor");
     return launch_ssg3d_velocity_spml(g, as_stream(stream));
 }
 
 int sw_ssg3d_stress_spml(const sw_ssg3d *g, void *stream)
 {
-    int rc = check_ssg3d(g, true);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
+    SW_REQUIRE(g->cls == SW_CLASS_SPML, "split-field stress sweep needs a split-field descriptor");
     return launch_ssg3d_stress_spml(g, as_stream(stream));
+}
+
+int sw_ssg3d_launch_plan(const sw_ssg3d *g, int32_t *pm, int32_t max)
+{
+    int rc = check_ssg3d(g);
+    if (rc) return rc;
+    SW_REQUIRE(max >= 0 && (max == 0 || pm), "null mask array");
+    return ssg3d_plan_masks(g, pm, max);
 }
 
 int sw_ssg3d_inject(const sw_ssg3d *g, const sw_source *src, int32_t it, void *stream)
@@ -468,8 +485,7 @@ int sw_ssg3d_run(const sw_ssg3d *g, const sw_source *src, const sw_receivers *re
 int sw_ssg3d_run_ex(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, int32_t it0,
                     int32_t nsteps, int32_t flags, void *stream)
 {
-    const bool spml = g && g->cls == SW_CLASS_SPML;
-    int rc = check_ssg3d(g, spml);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
     SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");
     if (nsteps == 0) return SW_OK;
@@ -834,7 +850,7 @@ int sw_band_columns(const sw_band *b)
 
 int sw_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, void *stream)
 {
-    int rc = check_ssg3d(g, g && g->cls == SW_CLASS_SPML);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
     if ((rc = check_band(g, b))) return rc;
     SW_REQUIRE(out != nullptr, "null output");
@@ -843,7 +859,7 @@ int sw_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, void *stream
 
 int sw_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, void *stream)
 {
-    int rc = check_ssg3d(g, g && g->cls == SW_CLASS_SPML);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
     if ((rc = check_band(g, b))) return rc;
     SW_REQUIRE(params && prev, "null network or band state");
@@ -853,9 +869,9 @@ int sw_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, fl
 int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, const sw_band *b,
                          const float *params, float *prev, int32_t it0, int32_t nsteps, void *stream)
 {
-    const bool spml = g && g->cls == SW_CLASS_SPML;
-    int rc = check_ssg3d(g, spml);
+    int rc = check_ssg3d(g);
     if (rc) return rc;
+    const bool spml = g->cls == SW_CLASS_SPML;
     if ((rc = check_band(g, b))) return rc;
     SW_REQUIRE(params && prev, "null network or band state");
     SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");

@@ -744,6 +744,17 @@ int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st
     return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d stress (split-field PML)" : "ssg3d stress");
 }
 
+int ssg3d_plan_masks(const sw_ssg3d *g, int32_t *pm, int32_t max)
+{
+    int n = 0;
+    SW_DISPATCH_ORDER(g->order, {
+        const Plan p = make_plan(*g, N);
+        n = p.n;
+        for (int t = 0; t < p.n && t < max; ++t) pm[t] = p.pm[t];
+    });
+    return n;
+}
+
 int launch_ssg3d_velocity(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_velocity_fold(g, Fold3D{}, st); }
 int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_stress_fold(g, Fold3D{}, st); }
 int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st) { return launch_ssg3d_velocity_fold(g, Fold3D{}, st); }

@@ -93,7 +93,7 @@ inline int launch_status(const char *what)
 }
 
 // Validation shared by the 3-D entry points.
-int check_ssg3d(const sw_ssg3d *g, bool spml);
+int check_ssg3d(const sw_ssg3d *g);  // class-specific boundary arrays follow g->cls
 int check_ssg2d(const sw_ssg2d *g);
 
 // Slab width of an axis: indices [0,lo) and [hi,n) carry memory variables.

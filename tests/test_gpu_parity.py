@@ -478,6 +478,65 @@ def test_driver_flow_with_taichi_stand_in_matches_oracle():
     _compare(w, o)
 
 
+_DRIVER = """
+import sys
+import numpy as np
+from src.Elastic3D_SSG_Tensor import ElasticWAVE
+nx = ny = nz = 40
+vp = np.full((nx, ny, nz), 3000., np.float32)
+vs = np.full((nx, ny, nz), 3000. / 1.67, np.float32)
+rho = np.full((nx, ny, nz), 2500., np.float32)
+dx = dy = dz = 2.5
+nt, accuracy, freq, dt, src_scale = 70, 3, 60., 3e-4, 1
+rsx = np.full((1, 13), 12, dtype=int)
+rsy = np.full((1, 13), 12, dtype=int)
+rsz = np.linspace(12, 24, 13, dtype=int).reshape(1, 13)
+MT = np.zeros((3, 3), np.float32)
+MT[1, 2] = MT[2, 1] = -1 / np.sqrt(2)
+test = ElasticWAVE(vs, vp, rho, dx, dy, dz, dt, 20, 20, 22, rsx, rsy, rsz, MT, src_scale, nt, accuracy, freq)
+pml_parameter = {"vp_max": 3000, "pml_x_thick": 8 * dx, "pml_y_thick": 8 * dy, "pml_z_thick": 8 * dz,
+                 "Rcoef": 0.000001, "theta": 1, "alpha_max_pml": np.pi * freq,
+                 "kmax_pml": (3000 / 1.67) / (5 * dx * freq)}
+test.SetADEPML3D([True, True, True, True, True, True], pml_parameter)
+for i in range(nt):
+    test.update_SSG(i)
+np.savez(sys.argv[1], data=test.data.to_numpy(), vx=test.vx.to_numpy(), szz=test.szz.to_numpy(),
+         module=type(test).__module__)
+"""
+
+
+def test_reference_driver_imports_src_package(tmp_path):
+    """A driver written as the reference's are (test3d_homegenous_tensor.py:4:
+    ``from src.Elastic3D_SSG_Tensor import ElasticWAVE``, then the reference
+    constructor, SetADEPML3D and update_SSG per step), run as its own Python
+    process with the repository root on sys.path and nothing else set up: the
+    import resolves to the package at the repository root and the run equals
+    the oracle bit for bit."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script, out = tmp_path / "driver.py", tmp_path / "out.npz"
+    script.write_text(_DRIVER)
+    env = dict(os.environ, PYTHONPATH=root)
+    r = subprocess.run([sys.executable, str(script), str(out)], cwd=str(tmp_path), env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = np.load(out)
+    assert str(got["module"]) == "seiswave_taichi_amd.Elastic3D_SSG_Tensor"
+    vs, vp, rho = (np.full((40, 40, 40), v, np.float32) for v in (3000 / 1.67, 3000.0, 2500.0))
+    d, dt, freq, nt = 2.5, 3e-4, 60.0, 70
+    rec = np.stack([np.full(13, 12), np.full(13, 12), np.linspace(12, 24, 13, dtype=int)], 1)
+    o = O.Oracle3D("tensor", vs, vp, rho, d, d, d, dt, (20, 20, 22), rec, nt, 3, freq=freq,
+                   mt=np.array([0, 0, 0, 0, 0, -1 / np.sqrt(2)], np.float32))
+    o.set_pml([True] * 6, pml_dict(3000.0, 8, (d, d, d), freq, 3000 / 1.67))
+    o.run(nt, amps=ricker2(np.arange(nt), dt, 0.03, freq, 1.0))
+    assert np.abs(o.data).max() > 0
+    np.testing.assert_array_equal(got["data"].reshape(o.data.shape), o.data)
+    np.testing.assert_array_equal(got["vx"], o.fields["vx"])
+    np.testing.assert_array_equal(got["szz"], o.fields["szz"])
+
+
 def _f64_report(gpu, o32, o64):
     """Per-trace and global relative L2 of the GPU (f32) and f32-oracle traces
     against the f64 oracle's."""

@@ -326,3 +326,64 @@ def test_grid_size_limits_are_checked():
     # on the next one (the unknown run flag)
     rc, msg = run(1100, 1000, 1000)
     assert rc == -1 and "unknown run flags" in msg
+
+
+def test_sweep_entry_points_refuse_the_other_class():
+    """Each 3-D sweep entry point validates the arrays of the class it runs:
+    the split-field sweeps refuse a CPML/Tensor descriptor (whose split arrays
+    were never checked) and the CPML sweeps a split-field one; an unknown
+    class is refused everywhere.  The checks return before any launch."""
+    import ctypes as C
+    lib = L.load()
+    g = _fake_3d()
+    for cls in (L.SW_CLASS_CPML, L.SW_CLASS_TENSOR):
+        g.cls = cls
+        for fn in (lib.sw_ssg3d_velocity_spml, lib.sw_ssg3d_stress_spml):
+            assert fn(C.byref(g), None) == -1
+            assert "split-field descriptor" in L.last_error()
+    g.cls = L.SW_CLASS_SPML
+    g.split = C.cast(C.c_void_p(0x1000), L._fp)
+    for ax in (g.px, g.py, g.pz):
+        ax.in_ = ax.in_h = C.cast(C.c_void_p(0x1000), L._u8p)
+    for fn in (lib.sw_ssg3d_velocity, lib.sw_ssg3d_stress):
+        assert fn(C.byref(g), None) == -1
+        assert "CPML/Tensor descriptor" in L.last_error()
+    # a split-field descriptor without split arrays fails the split-field check
+    g.split = None
+    assert lib.sw_ssg3d_velocity_spml(C.byref(g), None) == -1
+    assert "split-field arrays missing" in L.last_error()
+    g.cls = 7
+    for fn in (lib.sw_ssg3d_velocity, lib.sw_ssg3d_stress, lib.sw_ssg3d_velocity_spml, lib.sw_ssg3d_stress_spml):
+        assert fn(C.byref(g), None) == -1
+        assert "unknown 3-D class" in L.last_error()
+
+
+def test_launch_plan_rule():
+    """One launch with CPML on every axis while the CPML-free bulk is under
+    75 % of the updated cells, five (x; x, y twice; x, y, z twice) above."""
+    g = _fake_3d(n=60)
+    for ax in (g.px, g.py, g.pz):
+        ax.lo, ax.hi = 3, 57      # thin slabs: bulk (53/56)^2 = 0.90
+    assert L.launch_plan(g) == [1, 3, 3, 7, 7]
+    for ax in (g.px, g.py, g.pz):
+        ax.lo, ax.hi = 10, 50     # bulk (40/56)^2 = 0.51
+    assert L.launch_plan(g) == [7]
+
+
+def test_src_package_at_repository_root():
+    """``from src.<Module> import ElasticWAVE`` (test3d_homegenous_tensor.py:4,
+    test2d_homegenous.py:4, ...) resolves, in a fresh interpreter with only the
+    repository root on sys.path, to this package's classes."""
+    import subprocess
+    import sys
+    code = ("import src.Elastic2D_CPML as a, src.Elastic2D_sparse as b, src.Elastic3D_SSG_CPML as c, "
+            "src.Elastic3D_SSG_SPML as d, src.Elastic3D_SSG_Tensor as e\n"
+            "from src.wigb import *\nfrom src.BaseKernel import Ricker, Ricker2\n"
+            "print(' '.join(m.ElasticWAVE.__module__ for m in (a, b, c, d, e)), wigb.__module__)")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, PYTHONPATH=ROOT))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.split() == [f"seiswave_taichi_amd.{m}" for m in (
+        "Elastic2D_CPML", "Elastic2D_sparse", "Elastic3D_SSG_CPML", "Elastic3D_SSG_SPML",
+        "Elastic3D_SSG_Tensor", "wigb")]
+
