@@ -135,6 +135,41 @@ def copy_ceiling_gbs(gib=1.0, reps=5):
     return 2.0 * n * 4 / t / 1e9
 
 
+def stream_ceiling_gbs(n, reps=20):
+    """Rates of the library's streaming probe (sw_stream_probe: every element
+    of each array read or written once, 16-byte accesses) with the two
+    sweeps' array mixes on arrays of n floats, the practical HBM ceiling of a
+    sweep.  Several grid sizes are timed (HIP events on the launch stream);
+    the best counts.  Returns {mix: {"GBs": best, "by_blocks": {...}}}."""
+    import ctypes as C
+
+    import torch
+
+    from seiswave_taichi_amd import _lib as L
+    lib = L.load()
+    n = (n // 4) * 4
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for name, nin, nout in (("velocity_10r_3w", 10, 3), ("stress_11r_6w", 11, 6)):
+        bufs = [torch.ones(n, dtype=torch.float32, device="cuda") for _ in range(nin + nout)]
+        pin = (L._fp * nin)(*[L.fptr(b) for b in bufs[:nin]])
+        pout = (L._fp * nout)(*[L.fptr(b) for b in bufs[nin:]])
+        rates = {}
+        for nblock in (1024, 2048, 4096, 8192, (n // 4 + 255) // 256):
+            L.check(lib.sw_stream_probe(pin, nin, pout, nout, n, nblock, stream), "stream probe")
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                L.check(lib.sw_stream_probe(pin, nin, pout, nout, n, nblock, stream), "stream probe")
+            b.record()
+            torch.cuda.synchronize()
+            t = a.elapsed_time(b) / 1e3 / reps
+            rates[nblock] = round((nin + nout) * n * 4 / t / 1e9, 1)
+        out[name] = {"GBs": max(rates.values()), "by_blocks": rates}
+        del bufs
+    return out
+
+
 def cpu_baseline(w, budget_s=15.0):
     """Oracle (C restatement, OpenMP) on a bounded sample of the same workload."""
     from oracle import oracle as O
@@ -412,6 +447,11 @@ def main():
     ceiling = copy_ceiling_gbs()
     out["roofline"]["copy_ceiling_GBs"] = round(ceiling, 1)
     out["roofline"]["frac_of_copy_ceiling"] = round(ach / ceiling, 4)
+    streams = stream_ceiling_gbs(cells)
+    mix = streams["stress_11r_6w" if dom == "stress" else "velocity_10r_3w"]["GBs"]
+    out["roofline"]["stream_ceiling_GBs"] = mix
+    out["roofline"]["frac_of_stream_ceiling"] = round(ach / mix, 4)
+    out["stream_probe"] = streams
     if args.copy_calibration > 0:
         n = int(args.copy_calibration * (1 << 30)) // 4
         src_buf = torch.zeros(n, dtype=torch.float32, device="cuda")

@@ -305,6 +305,14 @@ int sw_mlp_loss_backward(const float *params, const float *X, const float *H1, c
                          const float *T, int32_t B, float *grad, float *work, float *loss_rows, void *stream);
 int sw_adam(float *p, const float *g, float *m, float *v, int32_t n, float lr, float beta1, float beta2, float eps,
             int32_t t, void *stream);
+/* ---- measurement (no counterpart in the reference) ----
+ * Streaming probe: out[o][e] = (o+1) * sum_q in[q][e] for e < n, every element
+ * read or written once with 16-byte accesses, nblock blocks of 256 threads
+ * in a grid-stride loop.  Instantiated mixes (nin read / nout written):
+ * 10/3 (the velocity sweep's arrays), 11/6 (the stress sweep's), 1/1 (a
+ * copy).  n a multiple of 4, arrays 16-byte aligned and device resident. */
+int sw_stream_probe(const float *const *in, int32_t nin, float *const *out, int32_t nout, int64_t n,
+                    int32_t nblock, void *stream);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,7 @@ SIGNATURES = {
     "sw_ssg3d_run_learned": [_P3, _PS, _PR, _PB, _fp, _fp, C.c_int32, C.c_int32, C.c_void_p],
     "sw_mlp_forward": [_fp, _fp, C.c_int32, _fp, _fp, _fp, C.c_void_p],
     "sw_mlp_loss_backward": [_fp] * 6 + [C.c_int32, _fp, _fp, _fp, C.c_void_p],
+    "sw_stream_probe": [C.POINTER(_fp), C.c_int32, C.POINTER(_fp), C.c_int32, C.c_int64, C.c_int32, C.c_void_p],
     "sw_adam": [_fp, _fp, _fp, _fp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
                 C.c_void_p],
 }

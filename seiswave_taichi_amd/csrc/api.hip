@@ -38,6 +38,8 @@ int launch_mlp_loss_backward(const float *params, const float *X, const float *H
 int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, int t,
                 hipStream_t st);
 int mlp_param_count();
+int launch_stream_probe(const float *const *in, int nin, float *const *out, int nout, long n, int nblock,
+                        hipStream_t st);
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
 int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st);
 int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st);
@@ -890,6 +892,19 @@ int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_recei
         if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
     }
     return launch_status("ssg3d learned run");
+}
+
+int sw_stream_probe(const float *const *in, int32_t nin, float *const *out, int32_t nout, int64_t n,
+                    int32_t nblock, void *stream)
+{
+    SW_REQUIRE(in && out && nin >= 1 && nout >= 1 && n > 0 && n % 4 == 0 && nblock > 0,
+               "bad stream probe arguments (n a positive multiple of 4)");
+    for (int q = 0; q < nin; ++q)
+        SW_REQUIRE(in[q] && reinterpret_cast<uintptr_t>(in[q]) % 16 == 0, "stream probe input %d not 16-byte aligned", q);
+    for (int o = 0; o < nout; ++o)
+        SW_REQUIRE(out[o] && reinterpret_cast<uintptr_t>(out[o]) % 16 == 0, "stream probe output %d not 16-byte aligned",
+                   o);
+    return launch_stream_probe(in, nin, out, nout, n, nblock, as_stream(stream));
 }
 
 int sw_mlp_forward(const float *params, const float *X, int32_t B, float *H1, float *H2, float *Y, void *stream)

@@ -1,0 +1,66 @@
+// Streaming probe: a kernel that moves exactly the bytes of a sweep's access
+// mix (nin arrays read, nout arrays written, every element once) and does no
+// stencil work, so its rate is the practical HBM ceiling the sweeps are
+// measured against.  No counterpart in the reference.
+//
+// Each thread handles kVec consecutive floats per array (16-byte loads and
+// stores) and walks the arrays with a grid stride; the host picks the grid.
+#include "sw_internal.h"
+
+namespace sw {
+
+namespace {
+
+constexpr int kMaxIn = 16, kMaxOut = 8;
+
+struct StreamArgs {
+    const float *in[kMaxIn];
+    float *out[kMaxOut];
+};
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int NIN, int NOUT>
+__global__ void __launch_bounds__(256) k_stream(StreamArgs a, long n4)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) acc += reinterpret_cast<const f4 *>(a.in[q])[t];
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) reinterpret_cast<f4 *>(a.out[o])[t] = acc * (float)(o + 1);
+    }
+}
+
+template <int NIN, int NOUT>
+void launch(const StreamArgs &a, long n4, int nblock, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_stream<NIN, NOUT>), dim3(nblock), dim3(256), 0, st, a, n4);
+}
+
+}  // namespace
+
+int launch_stream_probe(const float *const *in, int nin, float *const *out, int nout, long n, int nblock,
+                        hipStream_t st)
+{
+    StreamArgs a{};
+    for (int q = 0; q < nin; ++q) a.in[q] = in[q];
+    for (int o = 0; o < nout; ++o) a.out[o] = out[o];
+    const long n4 = n / 4;
+    // the mixes of the two sweeps (velocity: 10 read, 3 written; stress: 11
+    // read, 6 written) and of a copy
+    if (nin == 10 && nout == 3)
+        launch<10, 3>(a, n4, nblock, st);
+    else if (nin == 11 && nout == 6)
+        launch<11, 6>(a, n4, nblock, st);
+    else if (nin == 1 && nout == 1)
+        launch<1, 1>(a, n4, nblock, st);
+    else {
+        set_error("stream probe: mix %d read / %d written not instantiated (10/3, 11/6, 1/1)", nin, nout);
+        return SW_EINVAL;
+    }
+    return launch_status("stream probe");
+}
+
+}  // namespace sw
