@@ -305,6 +305,21 @@ int sw_mlp_loss_backward(const float *params, const float *X, const float *H1, c
                          const float *T, int32_t B, float *grad, float *work, float *loss_rows, void *stream);
 int sw_adam(float *p, const float *g, float *m, float *v, int32_t n, float lr, float beta1, float beta2, float eps,
             int32_t t, void *stream);
+/* sw_mlp_loss_backward without the work argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The dY/dZ2/dZ1 scratch is a static device array of
+ * SW_MLP_NOALLOC_MAX_ROWS * 164 floats (10.7 MB) that each device holds until
+ * the process ends; its address is looked up by sw_device_init or by the
+ * first call on a device.  B above SW_MLP_NOALLOC_MAX_ROWS is refused
+ * (SW_EINVAL).  Calls on one device share that scratch: calls that overlap
+ * on one GPU (concurrent streams) race; calls in one stream order do not.
+ * Nothing in this call allocates. */
+#define SW_MLP_NOALLOC_MAX_ROWS 16384
+int sw_mlp_loss_backward_noalloc(const float *params, const float *X, const float *H1, const float *H2,
+                                 const float *Y, const float *T, int32_t B, float *grad, float *loss_rows,
+                                 void *stream);
+
 /* ---- measurement (no counterpart in the reference) ----
  * Streaming probe: out[o][e] = (o+1) * sum_q in[q][e] for e < n, every element
  * read or written once with 16-byte accesses, nblock blocks of 256 threads

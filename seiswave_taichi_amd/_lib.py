@@ -17,6 +17,7 @@ SW_CLASS_CPML, SW_CLASS_TENSOR, SW_CLASS_SPML = 0, 1, 2
 SW_SRC_NONE, SW_SRC_FORCE_VZ, SW_SRC_MOMENT, SW_SRC_EXPLOSIVE, SW_SRC_FORCE_VX = 0, 1, 2, 3, 4
 SW_2D_VS, SW_2D_SV, SW_2D_RSG = 0, 1, 2
 SW_RUN_FREE_SURFACE = 1
+SW_MLP_NOALLOC_MAX_ROWS = 16384
 FIELD_CODE = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7, "syz": 8}
 
 _fp = C.POINTER(C.c_float)
@@ -112,6 +113,7 @@ SIGNATURES = {
     "sw_ssg3d_run_learned": [_P3, _PS, _PR, _PB, _fp, _fp, C.c_int32, C.c_int32, C.c_void_p],
     "sw_mlp_forward": [_fp, _fp, C.c_int32, _fp, _fp, _fp, C.c_void_p],
     "sw_mlp_loss_backward": [_fp] * 6 + [C.c_int32, _fp, _fp, _fp, C.c_void_p],
+    "sw_mlp_loss_backward_noalloc": [_fp] * 6 + [C.c_int32, _fp, _fp, C.c_void_p],
     "sw_stream_probe": [C.POINTER(_fp), C.c_int32, C.POINTER(_fp), C.c_int32, C.c_int64, C.c_int32, C.c_void_p],
     "sw_adam": [_fp, _fp, _fp, _fp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
                 C.c_void_p],

@@ -40,6 +40,10 @@ int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, f
 int mlp_param_count();
 int launch_stream_probe(const float *const *in, int nin, float *const *out, int nout, long n, int nblock,
                         hipStream_t st);
+int mlp_scratch_lookup(float **out);
+int launch_mlp_loss_backward_noalloc(const float *params, const float *X, const float *H1, const float *H2,
+                                     const float *Y, const float *T, int B, float *grad, float *loss_rows,
+                                     hipStream_t st);
 int launch_adj2d_velocity(const sw_ssg2d *f, const sw_adj2d *q, hipStream_t st);
 int launch_ssg2d_velocity_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st);
 int launch_ssg2d_stress_batch(const sw_ssg2d *g, int nbatch, long bs, const Fold2D &fo, hipStream_t st);
@@ -419,7 +423,8 @@ int sw_device_init(int device)
 {
     SW_TRY_HIP(hipSetDevice(device));
     SW_TRY_HIP(hipFree(nullptr));
-    return SW_OK;
+    float *scratch = nullptr;  // the allocation-free entry point's scratch address on this device
+    return mlp_scratch_lookup(&scratch);
 }
 
 int sw_ssg3d_velocity(const sw_ssg3d *g, void *stream)
@@ -919,6 +924,14 @@ int sw_mlp_loss_backward(const float *params, const float *X, const float *H1, c
     SW_REQUIRE(params && X && H1 && H2 && Y && T && grad && work && loss_rows && B > 0,
                "bad mlp backward arguments");
     return launch_mlp_loss_backward(params, X, H1, H2, Y, T, B, grad, work, loss_rows, as_stream(stream));
+}
+
+int sw_mlp_loss_backward_noalloc(const float *params, const float *X, const float *H1, const float *H2,
+                                 const float *Y, const float *T, int32_t B, float *grad, float *loss_rows,
+                                 void *stream)
+{
+    SW_REQUIRE(params && X && H1 && H2 && Y && T && grad && loss_rows && B > 0, "bad mlp backward arguments");
+    return launch_mlp_loss_backward_noalloc(params, X, H1, H2, Y, T, B, grad, loss_rows, as_stream(stream));
 }
 
 int sw_adam(float *p, const float *g, float *m, float *v, int32_t n, float lr, float beta1, float beta2, float eps,

@@ -21,6 +21,7 @@
 // (y = 0 where s = 0).  The training entry points see samples already divided
 // by s: their forward pass is the bracket applied to x / in_scale.
 #include <algorithm>
+#include <mutex>
 
 #include "sw_internal.h"
 
@@ -478,5 +479,52 @@ int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, f
 }
 
 int mlp_param_count() { return kParams; }
+
+// ------------------------------------------------ allocation-free training step
+// Scratch of sw_mlp_loss_backward_noalloc: the dY, dZ2 and dZ1 rows
+// (kOut + 2 kH floats per sample) of up to kScratchRows samples, 10.7 MB held
+// on every device the library runs on until the process ends.  In host code
+// `scratch` names a host-side stand-in; kernels get its device address from
+// mlp_scratch_lookup, one lookup per device.
+constexpr int kScratchRows = SW_MLP_NOALLOC_MAX_ROWS;
+constexpr long kScratchFloats = (long)kScratchRows * (kOut + 2 * kH);
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+int mlp_scratch_lookup(float **out)
+{
+    int dev = 0;
+    SW_TRY_HIP(hipGetDevice(&dev));
+    SW_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the scratch table holds", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        SW_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch[dev];
+    return SW_OK;
+}
+
+int launch_mlp_loss_backward_noalloc(const float *params, const float *X, const float *H1, const float *H2,
+                                     const float *Y, const float *T, int B, float *grad, float *loss_rows,
+                                     hipStream_t st)
+{
+    SW_REQUIRE(B <= kScratchRows, "batch of %d samples above the %d the allocation-free scratch holds", B,
+               kScratchRows);
+    float *work = nullptr;
+    int rc = mlp_scratch_lookup(&work);
+    if (rc) return rc;
+    // every scratch row is written (k_mse, k_dense_back) before it is read,
+    // so the previous call's contents need no clearing
+    return launch_mlp_loss_backward(params, X, H1, H2, Y, T, B, grad, work, loss_rows, st);
+}
 
 }  // namespace sw
