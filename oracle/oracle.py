@@ -46,6 +46,34 @@ def _lib(precision: str):
     path = os.path.join(_BUILD, f"liboracle_{precision}.so")
     if not os.path.exists(path):
         build()
+    _LIBS[precision] = _bind(path, precision)
+    return _LIBS[precision]
+
+
+class library:
+    """Context manager: oracle objects created inside use the oracle library at
+    ``path`` (built from a variant of seiswave_oracle.c) in place of the
+    committed build of that precision; used to check that the oracle's pins
+    reject a deliberately mistaken restatement."""
+
+    def __init__(self, path: str, precision: str = "f64"):
+        self.path, self.precision = path, precision
+
+    def __enter__(self):
+        self._saved = _LIBS.get(self.precision)
+        _lib(self.precision)  # the coefficient helpers keep using a real build
+        _LIBS[self.precision] = _bind(self.path, self.precision)
+        return self
+
+    def __exit__(self, *exc):
+        if self._saved is None:
+            _LIBS.pop(self.precision, None)
+        else:
+            _LIBS[self.precision] = self._saved
+        return False
+
+
+def _bind(path: str, precision: str):
     lib = C.CDLL(path)
     real = C.c_float if precision == "f32" else C.c_double
     P = C.POINTER(real)
@@ -102,7 +130,6 @@ def _lib(precision: str):
     lib.or_set_threads.restype = C.c_int
     lib.Axis, lib.State3D, lib.State2D, lib.real, lib.P = Axis, State3D, State2D, real, P
     lib.np_dtype = np.float32 if precision == "f32" else np.float64
-    _LIBS[precision] = lib
     return lib
 
 
@@ -268,6 +295,26 @@ class Oracle3D:
     def run(self, nsteps: int, it0: int = 0, amps=None):
         for n in range(nsteps):
             self.step(it0 + n, None if amps is None else float(amps[it0 + n]))
+
+    def step_moment(self, it: int, amp: float):
+        """One step of this class with the moment-tensor source of
+        src/Elastic3D_SSG_Tensor.py:186-193 in place of the class's own
+        source: s_ab[is] += amp*(-M_ab) before the velocity loops, then the
+        class's velocity and stress loops and the vx record (the order
+        sw_ssg3d_run uses for a stress source in every class; the reference
+        has a moment source only in the Tensor class)."""
+        if self.pml is None:
+            raise RuntimeError("SetADEPML3D must run before the first step")
+        if self.kind == "tensor":
+            return self.step(it, amp)
+        dt = self.lib.np_dtype
+        a = dt(amp)
+        for name, m in zip(("sxx", "syy", "szz", "sxy", "sxz", "syz"), self.mt):
+            self.fields[name][self.src] += a * (-m)
+        self.velocity()
+        self.stress()
+        for r, (i, j, k) in enumerate(self.rec):
+            self.data[it, r] = self.fields["vx"][i, j, k]
 
     def velocity(self):
         """The velocity loops alone (no source, no record)."""
