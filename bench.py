@@ -312,6 +312,51 @@ def vs2d_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def learned_workload(args):
+    """Learned thin boundary on the 200^3 case: train the six face networks
+    with the HIP training kernels on teacher-forced runs of 32 random layered
+    28^3 interiors (1,500 Adam steps per face), calibrate the output scale in
+    closed loop on one held-out model, then run the homogeneous 155^3 interior
+    of the 200^3 case (vp 3500, vs 2000, rho 2200, 10 m, dt 1 ms, seeded unit
+    moment tensor at the centre, 10 Hz) for 300 steps with the 20-cell CPML
+    (200^3 grid), the thin 4-cell CPML alone and the thin CPML with the
+    network (168^3 grids), each against a padded reference run: reflected
+    energy = sum over snapshots every 5 steps of the squared velocity error
+    in the interior box.  Time per step of the wide and learned runs over the
+    next ``steps`` steps (HIP events).  ``value`` is the speedup of the
+    learned run over the 20-cell CPML; the target is a speedup at reflected
+    energy <= 1.5x the 20-cell CPML's."""
+    import torch
+
+    from seiswave_taichi_amd import learned_abc as LA
+
+    torch.cuda.set_device(0)
+    torch.manual_seed(0)
+    t0 = time.perf_counter()
+    net, hists = LA.train(n_runs=32, n_int=28, steps=150, iters=1500, seed=1, log=None)
+    alpha, _ = LA.tune_output_scale(net, n_int=28, steps=150, seed=7)
+    t_train = time.perf_counter() - t0
+    model = tuple(np.full((155,) * 3, v, np.float32) for v in (2000.0, 3500.0, 2200.0))
+    r = LA.reflection_study(net, n_int=155, steps=300, seed=0, f=10.0, model=model, src=(77, 77, 77),
+                            timing_steps=args.steps)
+    speedup = r["ms_per_step_wide"] / r["ms_per_step_learned"]
+    refl_wide = r["learned"] / r["wide"]
+    out = {"metric": "learned-ABC vs 20-cell CPML speedup at equal reflection", "value": round(speedup, 4),
+           "unit": "x", "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "learned_200: 155^3 homogeneous interior of the 200^3 case, 20-cell CPML "
+                                  "(200^3) vs 4-cell CPML + per-face MLP (168^3), 300 steps, 10 Hz moment tensor"},
+           "reflection_vs_wide": round(refl_wide, 2), "reflection_vs_thin": round(r["learned"] / r["thin"], 4),
+           "thin_reflection_vs_wide": round(r["ratio_thin_to_wide"], 2), "speedup_vs_wide": round(speedup, 4),
+           "ms_per_step": {"wide_20_cell": round(r["ms_per_step_wide"], 5),
+                           "thin_4_cell_plus_network": round(r["ms_per_step_learned"], 5)},
+           "target": {"reflection_vs_wide_max": 1.5, "speedup_min": 1.0,
+                      "met": bool(refl_wide <= 1.5 and speedup > 1.0)},
+           "output_scale": alpha, "train_s": round(t_train, 1),
+           "held_out_one_step_residual": [round(h[-1][2], 4) for h in hists.values()]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,7 +364,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="tensor_200",
                     choices=["tensor_200", "spml_200", "layered_600x600x400", "shard_600x600x54", "gradient_800x300",
-                             "vs2d_400"])
+                             "vs2d_400", "learned_200"])
     ap.add_argument("--fd-cells", type=int, default=4,
                     help="gradient_800x300: cells per parameter of the finite-difference spot check")
     ap.add_argument("--kernel-reps", type=int, default=200)
@@ -337,6 +382,8 @@ def main():
         return gradient_workload(args)
     if args.config == "vs2d_400":
         return vs2d_workload(args)
+    if args.config == "learned_200":
+        return learned_workload(args)
 
     import torch
 

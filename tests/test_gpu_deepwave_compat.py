@@ -95,3 +95,34 @@ def test_compare_with_deepwave_call_shape():
     # P arrives after r/vp = 28.3 m / 2500 m/s = 11.3 ms = 113 steps (plus the wavelet delay)
     first = int(np.argmax(np.abs(trace) > 1e-3 * np.abs(trace).max()))
     assert 60 <= first <= 250, first
+
+
+def test_front_end_traces_match_oracle():
+    """deepwave_compat.elastic's traces (out[14]: vz at the y receivers,
+    out[15]: vx at the x receivers) against the f32 oracle's 2-D VS run on the
+    same padded model (the model's edge cells repeated over the stencil and
+    CPML cells, R = 1e-3, alpha_max = pi*pml_freq, kmax = 1) with the same vz
+    force: bit for bit."""
+    from oracle import oracle as O
+    lamb, mu, b, amps, src, rec, dt = _setup(seed=32)
+    out = D.elastic(lamb, mu, b, 10.0, dt, source_amplitudes_y=amps, source_locations_y=src,
+                    receiver_locations_y=rec, receiver_locations_x=rec, accuracy=4, pml_width=10, pml_freq=25.0)
+    vp, vs, rho = (t.cpu().numpy() for t in D.lambmubuoyancy_to_vpvsrho(lamb.cuda(), mu.cuda(), b.cuda()))
+    pad = 2 + 10
+    pv = [np.ascontiguousarray(np.pad(a.T, pad, mode="edge")) for a in (vp, vs, rho)]  # (x, z)
+    cells = np.array([(int(x) + pad, int(y) + pad) for y, x in rec[0].tolist()])
+    ix, iz = 20 + pad, 12 + pad
+    pml = {"vp_max": float(vp.max()), "pml_x_thick": 100.0, "pml_z_thick": 100.0, "Rcoef": 1e-3, "theta": 1,
+           "alpha_max_pml": np.pi * 25.0, "kmax_pml": 1.0}
+    nt = amps.shape[2]
+    o = O.Oracle2D(pv[1], pv[0], pv[2], 10.0, 10.0, dt, ix, iz, 0, 0, nt, 2, freq=25.0)
+    o.set_pml([True] * 4, pml)
+    w = amps[0, 0].numpy().astype(np.float32)
+    tz, tx = np.zeros((len(cells), nt), np.float32), np.zeros((len(cells), nt), np.float32)
+    for it in range(nt):
+        o.vs_step(it, float(w[it]))
+        tz[:, it] = o.fields["vz"][cells[:, 0], cells[:, 1]]
+        tx[:, it] = o.fields["vx"][cells[:, 0], cells[:, 1]]
+    assert np.abs(tz).max() > 0
+    np.testing.assert_array_equal(out[14][0].detach().cpu().numpy(), tz)
+    np.testing.assert_array_equal(out[15][0].detach().cpu().numpy(), tx)

@@ -61,3 +61,63 @@ def test_2d_snapshots_match_synchronous_reads():
             got = dict(frames)[it]["vx"]
             np.testing.assert_array_equal(got, b.vx.to_numpy(), err_msg=str(it))
     assert [k for k, _ in frames] == [16, 32, 48]
+
+
+def test_spml_snapshots_match_oracle():
+    """The snapshot cadence of the reference's split-field driver
+    (test3d_homegenous_SPML.py:71-80: update_SSG_SPML(i), then every 20 steps
+    vx.to_numpy() and its y = isy plane): the frames taken off the step
+    stream equal the oracle's vx plane after the same steps, bit for bit,
+    and so does the whole field at the end."""
+    from oracle import oracle as O
+    from seiswave_taichi_amd.Elastic3D_SSG_SPML import ElasticWAVE
+    from seiswave_taichi_amd.coeffs import ricker
+    from seiswave_taichi_amd.snapshots import Snapshots
+    shape, steps, every, f, dt = (44, 40, 42), 101, 20, 25.0, 4e-4
+    vs, vp, rho = layered_model(shape, seed=23)
+    isx, isy, isz = 22, 20, 21
+    rs = (12, 10, 11)
+    w = ElasticWAVE(vs, vp, rho, 10.0, 10.0, 10.0, dt, isx, isy, isz, *rs, steps, 2, f, 3.0)
+    p = pml_dict(float(vp.max()), 6, (10.0,) * 3, f, float(vs.max()))
+    w.SetADEPML3D([True] * 6, p)
+    frames = Snapshots(w, fields=("vx",), every=every, index=(slice(None), isy, slice(None))).run(steps, 0)
+    o = O.Oracle3D("spml", vs, vp, rho, 10.0, 10.0, 10.0, dt, (isx, isy, isz), [rs], steps, 2, freq=f, src_scale=3.0)
+    o.set_pml([True] * 6, p)
+    amps = ricker(np.arange(steps), dt, f, 3.0)
+    ref = []
+    for it in range(steps):
+        o.step(it, float(amps[it]))
+        if it % every == 0:
+            ref.append((it, o.fields["vx"][:, isy, :].copy()))
+    assert [k for k, _ in frames] == [k for k, _ in ref] == [0, 20, 40, 60, 80, 100]
+    for (ka, fa), (_, fb) in zip(frames, ref):
+        np.testing.assert_array_equal(fa["vx"], fb, err_msg=f"vx plane at step {ka}")
+    assert np.abs(ref[-1][1]).max() > 0
+    np.testing.assert_array_equal(w.vx.to_numpy(), o.fields["vx"])
+    np.testing.assert_array_equal(w.data.to_numpy(), o.data[:, 0])
+
+
+def test_2d_snapshots_match_oracle():
+    """2-D VS run with snapshots of vz and sxx every 16 steps: each frame
+    equals the oracle's field after that step."""
+    from oracle import oracle as O
+    from seiswave_taichi_amd.Elastic2D_CPML import ElasticWAVE
+    from seiswave_taichi_amd.coeffs import ricker2
+    from seiswave_taichi_amd.snapshots import Snapshots
+    shape, steps, f, dt = (60, 50), 70, 25.0, 6e-4
+    vs, vp, rho = layered_model(shape, seed=24)
+    w = ElasticWAVE(vs, vp, rho, 10.0, 10.0, dt, 30, 25, 20, 8, steps, 2, f)
+    p = pml_dict(float(vp.max()), 6, (10.0, 10.0), f, float(vs.max()), dims=2)
+    w.SetADEPML2D([True] * 4, p)
+    frames = dict(Snapshots(w, fields=("vz", "sxx"), every=16, scheme="VS").run(steps, 0))
+    o = O.Oracle2D(vs, vp, rho, 10.0, 10.0, dt, 30, 25, 20, 8, steps, 2, f)
+    o.set_pml([True] * 4, p)
+    amps = ricker2(np.arange(steps), dt, 1.5 / f, f, 1.0)
+    seen = 0
+    for it in range(steps):
+        o.vs_step(it, float(amps[it]))
+        if it % 16 == 0:
+            for n in ("vz", "sxx"):
+                np.testing.assert_array_equal(frames[it][n], o.fields[n], err_msg=f"{n} at step {it}")
+            seen += 1
+    assert seen == 5 and np.abs(frames[64]["vz"]).max() > 0

@@ -1,26 +1,63 @@
 """Learned thin boundary end to end: teacher-forced data from the plain
-solver, training with the HIP kernels, and the reflected energy in closed loop
-on a model not used for training, against the same thin CPML without the
-network."""
+solver, training with the HIP kernels, closed-loop calibration on one
+held-out model, then evaluation on models that played no part in training
+or calibration, and on the 200^3 case against the 20-cell CPML."""
+import numpy as np
 import pytest
 import torch
 
 pytestmark = pytest.mark.gpu
 
 
-def test_trained_network_beats_plain_thin_cpml():
-    """Train on teacher-forced runs of random models; then
-    (a) on held-out runs the calibrated face networks leave less one-step
-        boundary error than no network, on average over the faces, and never
-        more on any face (a face whose correction does not help is scaled to
-        zero);
-    (b) in closed loop on a further model, not used for training, the thin
-        CPML with the network reflects less energy than the same thin CPML
-        without it (same grid, same steps) at the calibrated output scale."""
+@pytest.fixture(scope="module")
+def trained():
     from seiswave_taichi_amd import learned_abc as LA
     torch.manual_seed(0)
     net, hists = LA.train(n_runs=32, n_int=28, steps=150, iters=1500, seed=1, log=None)
-    final = [h[-1][2] for h in hists.values()]  # after calibration
-    assert max(final) <= 1.0 + 1e-9 and sum(final) / len(final) < 1.0, final
     alpha, energies = LA.tune_output_scale(net, n_int=28, steps=150, seed=7)
-    assert alpha > 0.0 and energies[alpha] < energies[0.0], energies
+    return net, hists, alpha, energies
+
+
+def test_held_out_one_step_gain(trained):
+    """On held-out teacher-forced runs the calibrated face networks leave less
+    one-step boundary error than no network, on average over the faces, and
+    never more on any face (a face whose correction does not help is scaled
+    to zero); the closed-loop calibration keeps a nonzero output scale."""
+    _, hists, alpha, _ = trained
+    final = [h[-1][2] for h in hists.values()]
+    assert max(final) <= 1.0 + 1e-9 and sum(final) / len(final) < 1.0, final
+    assert alpha > 0.0
+
+
+def test_closed_loop_on_unseen_models(trained):
+    """Six random layered models that played no part in training or in
+    choosing the output scale (seeds 11-16): the thin CPML with the network
+    reflects less energy than the same thin CPML without it on most of them
+    (measured: 4 of 6 lower, ratios 0.79-1.00, and 1.51 on one), and its
+    reflected energy stays within 2x of the thin layer's on every one."""
+    from seiswave_taichi_amd import learned_abc as LA
+    net = trained[0]
+    ratios = []
+    for seed in range(11, 17):
+        r = LA.reflection_study(net, n_int=28, steps=150, seed=seed)
+        ratios.append(r["learned"] / r["thin"])
+    print("learned/thin on unseen models:", [round(x, 4) for x in ratios])
+    assert sum(x < 1.0 for x in ratios) >= 4, ratios
+    assert max(ratios) < 2.0, ratios
+
+
+def test_config4_evaluation_on_the_200_case(trained):
+    """The 200^3 case (155^3 homogeneous interior, 10 Hz moment source, 300
+    steps): the thin CPML with the network runs faster per step than the
+    20-cell CPML.  Its reflected energy is recorded, not asserted against the
+    1.5x target, which it misses (measured: about 3e3x the 20-cell layer's and
+    2.7x the thin layer's alone)."""
+    from seiswave_taichi_amd import learned_abc as LA
+    net = trained[0]
+    model = tuple(np.full((155,) * 3, v, np.float32) for v in (2000.0, 3500.0, 2200.0))
+    r = LA.reflection_study(net, n_int=155, steps=300, seed=0, f=10.0, model=model, src=(77, 77, 77),
+                            timing_steps=200)
+    print({k: float(v) for k, v in r.items()})
+    assert all(np.isfinite(v) for v in r.values())
+    assert r["wide"] > 0 and r["thin"] > r["wide"]
+    assert r["ms_per_step_learned"] < r["ms_per_step_wide"], r
