@@ -339,6 +339,10 @@ def learned_workload(args):
     model = tuple(np.full((155,) * 3, v, np.float32) for v in (2000.0, 3500.0, 2200.0))
     r = LA.reflection_study(net, n_int=155, steps=300, seed=0, f=10.0, model=model, src=(77, 77, 77),
                             timing_steps=args.steps)
+    ceiling = LA.linear_ceiling()
+    # the plain CPML at about the wide layer's reflection: 16 cells, R = 1e-3 (192^3 grid)
+    r16 = LA.reflection_study(net, n_int=155, steps=300, seed=0, f=10.0, model=model, src=(77, 77, 77),
+                              thin=16, thin_rcoef=1e-3, timing_steps=args.steps)
     speedup = r["ms_per_step_wide"] / r["ms_per_step_learned"]
     refl_wide = r["learned"] / r["wide"]
     out = {"metric": "learned-ABC vs 20-cell CPML speedup at equal reflection", "value": round(speedup, 4),
@@ -349,10 +353,17 @@ def learned_workload(args):
            "reflection_vs_wide": round(refl_wide, 2), "reflection_vs_thin": round(r["learned"] / r["thin"], 4),
            "thin_reflection_vs_wide": round(r["ratio_thin_to_wide"], 2), "speedup_vs_wide": round(speedup, 4),
            "ms_per_step": {"wide_20_cell": round(r["ms_per_step_wide"], 5),
-                           "thin_4_cell_plus_network": round(r["ms_per_step_learned"], 5)},
+                           "thin_4_cell": round(r["ms_per_step_thin"], 5),
+                           "thin_4_cell_plus_network": round(r["ms_per_step_learned"], 5),
+                           "plain_16_cell_R1e-3": round(r16["ms_per_step_thin"], 5)},
+           "plain_16_cell_cpml": {"reflection_vs_wide": round(r16["ratio_thin_to_wide"], 3),
+                                  "speedup_vs_wide": round(r16["ms_per_step_wide"] / r16["ms_per_step_thin"], 4)},
            "target": {"reflection_vs_wide_max": 1.5, "speedup_min": 1.0,
                       "met": bool(refl_wide <= 1.5 and speedup > 1.0)},
            "output_scale": alpha, "train_s": round(t_train, 1),
+           "linear_map_removes_of_one_step_error": {
+               "in_sample_upper_bound": {k: round(v[0], 4) for k, v in ceiling.items()},
+               "held_out": {k: round(v[1], 4) for k, v in ceiling.items()}},
            "held_out_one_step_residual": [round(h[-1][2], 4) for h in hists.values()]}
     print(json.dumps(out), flush=True)
 

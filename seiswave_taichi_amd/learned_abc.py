@@ -161,7 +161,7 @@ def pml_rcoef(npml):
     return THIN_RCOEF if npml <= THIN_MAX else WIDE_RCOEF
 
 
-def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
+def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f, rcoef=None):
     from .Elastic3D_SSG_Tensor import ElasticWAVE
     cr = courant(vp.max(), d, dt)
     if cr >= 0.95:
@@ -172,7 +172,8 @@ def _sim(vs, vp, rho, d, dt, cell, mt, amps, npml, f):
                     np.zeros((1, 1), np.int64) + cell[1], np.zeros((1, 1), np.int64) + cell[2], mt3, 1.0, nt, 2, f,
                     source={"kind": "moment", "cell": cell, "mt": mt, "wavelet": amps})
     p = {"vp_max": float(vp.max()), "pml_x_thick": npml * d, "pml_y_thick": npml * d, "pml_z_thick": npml * d,
-         "Rcoef": pml_rcoef(npml), "theta": 1, "alpha_max_pml": np.pi * PML_FREQ, "kmax_pml": 1.0}
+         "Rcoef": pml_rcoef(npml) if rcoef is None else rcoef, "theta": 1, "alpha_max_pml": np.pi * PML_FREQ,
+         "kmax_pml": 1.0}
     w.SetADEPML3D([True] * 6, p)
     return w
 
@@ -395,6 +396,38 @@ def train(n_runs=16, n_int=40, steps=160, iters=3000, seed=0, log=print, n_val=N
     return net, hists
 
 
+def linear_ceiling(n_runs=16, n_val=4, n_int=28, steps=150, seed=3):
+    """Share of the one-step band error that the best linear map of the same
+    72 features removes, per face: least squares T ~ X W (no bias: the wave
+    equation and the CPML are linear), in f64 on teacher-forced runs.
+    Returns {face: (in_sample, held_out)}: ``in_sample`` fits and evaluates
+    on all ``n_runs`` runs, an upper bound on what any linear map removes
+    from those samples; ``held_out`` fits on the first n_runs - n_val runs
+    and evaluates on the rest (negative when the fit makes the held-out error
+    larger than no correction).  removed = 1 - |T - X W|^2 / |T|^2."""
+    rng = np.random.default_rng(seed)
+    pools = {f: ([], []) for f in range(6)}
+    for _ in range(n_runs):
+        data = collect_run(rng, n_int=n_int, steps=steps)
+        for f in range(6):
+            pools[f][0].append(data[f][0])
+            pools[f][1].append(data[f][1])
+
+    def removed(X, T, Xe, Te):
+        W = torch.linalg.lstsq(X, T).solution
+        return float(1.0 - torch.sum((Te - Xe @ W) ** 2) / torch.sum(Te ** 2).clamp_min(1e-300))
+
+    out = {}
+    nt = n_runs - n_val
+    for f in range(6):
+        X = [x.double() for x in pools[f][0]]
+        T = [t.double() for t in pools[f][1]]
+        Xa, Ta = torch.cat(X), torch.cat(T)
+        out[FACES[f]] = (removed(Xa, Ta, Xa, Ta),
+                         removed(torch.cat(X[:nt]), torch.cat(T[:nt]), torch.cat(X[nt:]), torch.cat(T[nt:])))
+    return out
+
+
 # ---------------------------------------------------------------- inference
 class LearnedRun:
     """Runs an ElasticWAVE (3-D) instance with the learned band correction."""
@@ -436,7 +469,7 @@ def tune_output_scale(net: BandNet, alphas=(1.0, 0.5, 0.25, 0.1, 0.05), n_int=28
 
 
 def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20,
-                     model=None, src=None, every=5, timing_steps=0):
+                     model=None, src=None, every=5, timing_steps=0, thin_rcoef=None):
     """Reflected energy of thin CPML (with and without the network) and of the
     wide CPML, against a padded reference whose own boundary reflections
     cannot reach the interior within ``steps``.
@@ -444,8 +477,9 @@ def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3,
     ``model`` is (vs, vp, rho) over the n_int^3 interior; every run extends it
     outwards by edge replication.  Energy = sum over snapshots every ``every``
     steps and over the interior box of (v - v_ref)^2 for the three velocity
-    components.  ``timing_steps`` > 0 also times the thin+network and the wide
-    runs (ms per step, HIP events)."""
+    components.  ``timing_steps`` > 0 also times the thin (alone and with
+    the network) and the wide runs (ms per step, HIP events).  ``thin_rcoef``
+    overrides the thin layer's target reflection coefficient."""
     rng = np.random.default_rng(seed)
     if model is None:
         model = random_layered_model((n_int,) * 3, rng)
@@ -462,7 +496,8 @@ def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3,
         lo = 2 + npml + extra  # first interior index on every axis
         mdl = tuple(np.ascontiguousarray(np.pad(np.asarray(a, np.float32), ((lo, lo + 1),) * 3, mode="edge"))
                     for a in model)
-        sim = _sim(*mdl, d, dt, tuple(c + lo for c in src), m, amps, npml, f)
+        rc = thin_rcoef if name in ("thin", "learned") else None
+        sim = _sim(*mdl, d, dt, tuple(c + lo for c in src), m, amps, npml, f, rc)
         runs[name] = (sim, lo)
     learned = LearnedRun(runs["learned"][0], net)
     energy = {k: 0.0 for k in ("wide", "thin", "learned")}
@@ -488,7 +523,7 @@ def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3,
     out["ratio_learned_to_wide"] = energy["learned"] / max(energy["wide"], 1e-300)
     out["ratio_thin_to_wide"] = energy["thin"] / max(energy["wide"], 1e-300)
     if timing_steps > 0:
-        for name in ("wide", "learned"):
+        for name in ("wide", "thin", "learned"):
             sim, _ = runs[name]
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
