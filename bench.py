@@ -106,6 +106,43 @@ def algorithmic_bytes(shape, s, slabs, spml=False):
     return n_u, s_u, vel, stress
 
 
+def split_component_bytes(e, n_u, s=2):
+    """Compulsory bytes per sweep of the split-field class: the plain count
+    (52 / 68 B per updated cell) plus, for every component and cell where the
+    component's in-PML test holds (the OR over axes of the flag at its own
+    stagger), its split values read and written instead of its old value:
+    +20 B for a velocity or normal-stress component (3 splits), +12 B for a
+    shear stress (2 splits).  Cells are counted over the updated core
+    [s, n - s - 1) of every axis, as N_u is."""
+    flags = []
+    for ax, n in zip(e.pml, e.gridsize):
+        core = slice(s, n - s - 1)
+        flags.append((np.asarray(ax.inside, bool)[core], np.asarray(ax.inside_h, bool)[core]))
+
+    def count(hx, hy, hz):  # cells where the x, y or z flag (integer 0 / half 1) holds
+        fx, fy, fz = flags[0][hx], flags[1][hy], flags[2][hz]
+        return int(fx.size * fy.size * fz.size - (~fx).sum() * (~fy).sum() * (~fz).sum())
+
+    vel = 52 * n_u + 20 * (count(1, 0, 0) + count(0, 1, 0) + count(0, 0, 1))
+    stress = 68 * n_u + 20 * 3 * count(0, 0, 0) + 12 * (count(1, 1, 0) + count(1, 0, 1) + count(0, 1, 1))
+    # cells with some flag set: the shell, whose cells move the split mixes
+    flags = [(fi | fh, None) for fi, fh in flags]
+    shell = count(0, 0, 0)
+    return {"velocity": vel, "stress": stress, "shell_cells": shell}
+
+
+def blended_ceiling(streams, b_split, n_u, dom):
+    """Streaming rate of the split-field sweep's mix: its plain cells at the
+    plain mix's probe rate, its shell cells (all split values read and
+    written) at the split mix's, weighted by their bytes."""
+    shell = b_split["shell_cells"]
+    per = {"velocity": (52, 112, "velocity_10r_3w", "split_velocity_16r_12w"),
+           "stress": (68, 164, "stress_11r_6w", "split_stress_20r_21w")}[dom]
+    b_plain, b_shell = per[0] * (n_u - shell), per[1] * shell
+    t = b_plain / streams[per[2]]["GBs"] + b_shell / streams[per[3]]["GBs"]
+    return round((b_plain + b_shell) / t, 1)
+
+
 def traffic_from_profile(kernel_tag, workload_name):
     """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{workload_name}.json")
@@ -135,12 +172,16 @@ def copy_ceiling_gbs(gib=1.0, reps=5):
     return 2.0 * n * 4 / t / 1e9
 
 
-def stream_ceiling_gbs(n, reps=20):
+STREAM_MIXES = (("velocity_10r_3w", 10, 3), ("stress_11r_6w", 11, 6))
+SPLIT_SHELL_MIXES = (("split_velocity_16r_12w", 16, 12), ("split_stress_20r_21w", 20, 21))
+
+
+def stream_ceiling_gbs(n, reps=20, mixes=STREAM_MIXES):
     """Rates of the library's streaming probe (sw_stream_probe: every element
-    of each array read or written once, 16-byte accesses) with the two
-    sweeps' array mixes on arrays of n floats, the practical HBM ceiling of a
-    sweep.  Several grid sizes are timed (HIP events on the launch stream);
-    the best counts.  Returns {mix: {"GBs": best, "by_blocks": {...}}}."""
+    of each array read or written once, 16-byte accesses) with the sweeps'
+    array mixes on arrays of n floats, the practical HBM ceiling of a sweep.
+    Several grid sizes are timed (HIP events on the launch stream); the best
+    counts.  Returns {mix: {"GBs": best, "by_blocks": {...}}}."""
     import ctypes as C
 
     import torch
@@ -150,7 +191,7 @@ def stream_ceiling_gbs(n, reps=20):
     n = (n // 4) * 4
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     out = {}
-    for name, nin, nout in (("velocity_10r_3w", 10, 3), ("stress_11r_6w", 11, 6)):
+    for name, nin, nout in mixes:
         bufs = [torch.ones(n, dtype=torch.float32, device="cuda") for _ in range(nin + nout)]
         pin = (L._fp * nin)(*[L.fptr(b) for b in bufs[:nin]])
         pout = (L._fp * nout)(*[L.fptr(b) for b in bufs[nin:]])
@@ -487,29 +528,30 @@ def main():
         "bytes_per_step": step_bytes, "N_u": n_u, "S_u": s_u,
     }
     if spml:
-        # The split-field class updates all 24 split components in every cell
-        # with any in-PML flag (src/Elastic3D_SSG_SPML.py:182-391), not only the
-        # components along damped directions that the per-pair count assumes:
-        # 9 (velocity) and 15 (stress) components read and written per such cell.
-        from seiswave_taichi_amd._engine3d import split_free_range
-        free = 1
-        for ax, n in zip(e.pml, e.gridsize):
-            lo, hi = split_free_range(ax)
-            free *= max(0, min(hi, n - 3) - max(lo, 2))
-        n_split = n_u - free
-        b_all = {"velocity": 52 * n_u + 72 * n_split, "stress": 68 * n_u + 120 * n_split}
-        a_all = b_all[dom] / kt[dom] / 1e9
-        out["roofline_all_split_components"] = {
-            "split_cells": int(n_split), "algorithmic_bytes_per_launch": int(b_all[dom]),
-            "achieved": round(a_all, 1), "frac": round(a_all / HBM_PEAK_GBS, 4)}
+        # The split-field class updates a component from its split values
+        # wherever the component's own in-PML test holds
+        # (src/Elastic3D_SSG_SPML.py:182-391), in all its split directions,
+        # not only the damped ones the per-pair count assumes: there the old
+        # value is not read and 3 (normal components) or 2 (shears) split
+        # values are read and written, +20 B or +12 B per such cell.
+        b_split = split_component_bytes(e, n_u)
+        a_split = b_split[dom] / kt[dom] / 1e9
+        out["roofline_split_components"] = {
+            "algorithmic_bytes_per_launch": int(b_split[dom]), "shell_cells": b_split["shell_cells"],
+            "algorithmic_bytes_by_sweep": {k: int(b_split[k]) for k in ("velocity", "stress")},
+            "achieved": round(a_split, 1), "frac": round(a_split / HBM_PEAK_GBS, 4)}
     ceiling = copy_ceiling_gbs()
     out["roofline"]["copy_ceiling_GBs"] = round(ceiling, 1)
     out["roofline"]["frac_of_copy_ceiling"] = round(ach / ceiling, 4)
-    streams = stream_ceiling_gbs(cells)
+    streams = stream_ceiling_gbs(cells, mixes=STREAM_MIXES + (SPLIT_SHELL_MIXES if spml else ()))
     mix = streams["stress_11r_6w" if dom == "stress" else "velocity_10r_3w"]["GBs"]
     out["roofline"]["stream_ceiling_GBs"] = mix
     out["roofline"]["frac_of_stream_ceiling"] = round(ach / mix, 4)
     out["stream_probe"] = streams
+    if spml:
+        rs = out["roofline_split_components"]
+        rs["stream_ceiling_GBs"] = blended_ceiling(streams, b_split, n_u, dom)
+        rs["frac_of_stream_ceiling"] = round(rs["achieved"] / rs["stream_ceiling_GBs"], 4)
     if args.copy_calibration > 0:
         n = int(args.copy_calibration * (1 << 30)) // 4
         src_buf = torch.zeros(n, dtype=torch.float32, device="cuda")

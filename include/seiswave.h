@@ -73,9 +73,27 @@ typedef struct sw_axis {
  *   y: sxy_dy syy_dy syz_dy dvy_dy dvx_dy dvz_dy
  * mem_z holds 6 arrays [wz][ny][nx]:
  *   z: sxz_dz syz_dz szz_dz dvz_dz dvx_dz dvy_dz
- * split (SPML only) holds 24 full-grid arrays in the order
- *   vx_x vx_y vx_z vy_x vy_y vy_z vz_x vz_y vz_z sxx_x sxx_y sxx_z
- *   syy_x syy_y syy_z sxy_x sxy_y sxz_x sxz_z syz_y syz_z szz_x szz_y szz_z */
+ * split (SPML only, 16-byte aligned) holds the 24 split components
+ *   c:  0    1    2    3    4    5    6    7    8    9     10    11
+ *       vx_x vx_y vx_z vy_x vy_y vy_z vz_x vz_y vz_z sxx_x sxx_y sxx_z
+ *   c:  12    13    14    15    16    17    18    19    20    21    22    23
+ *       syy_x syy_y syy_z sxy_x sxy_y sxz_x sxz_z syz_y syz_z szz_x szz_y szz_z
+ * of the cells outside the split-free box [px.lo,px.hi) x [py.lo,py.hi) x
+ * [pz.lo,pz.hi) only (the reference's full-grid split arrays are exactly 0
+ * inside it).  Those S = sw_ssg3d_split_cells(g) cells are numbered (their
+ * "slot") through three parts, with w_d = n_d - (hi_d - lo_d),
+ * m_d = hi_d - lo_d and the slab index s(i) = i < lo ? i : i - (hi - lo):
+ *   planes k outside [pz.lo,pz.hi):          (s(k)*ny + j)*nx + i
+ *   then planes inside, rows outside:         (k-pz.lo)*wy*nx + s(j)*nx + i
+ *   then those planes, rows inside, columns
+ *   outside [px.lo,px.hi):                    (k-pz.lo)*my*wx + (j-py.lo)*wx + s(i)
+ * (each part after the previous one's last slot).  The slots come in blocks
+ * of 64, block b holding floats [1536b, 1536b + 1536) of split; component c
+ * of slot 64b + l sits at float 1536b + off + w*l + (c - c0) of the group
+ * (c0..c0+w-1 at offset off) that holds it:
+ *   c 0-3 at 0, 4-7 at 256, 9-12 at 576, 13-16 at 832, 17-20 at 1088 (w 4);
+ *   c 21-22 at 1344 (w 2); c 8 at 512, c 23 at 1472 (w 1).
+ * Storage: 1536 * ceil(S / 64) floats. */
 typedef struct sw_ssg3d {
     int32_t nx, ny, nz, order; /* order = accuracy N (half stencil), 1..SW_MAX_ORDER */
     int32_t cls;               /* SW_CLASS_* */
@@ -134,6 +152,18 @@ int sw_ssg3d_stress(const sw_ssg3d *g, void *stream);
 /* Split-field velocity / stress loops: src/Elastic3D_SSG_SPML.py:182-259, :296-391. */
 int sw_ssg3d_velocity_spml(const sw_ssg3d *g, void *stream);
 int sw_ssg3d_stress_spml(const sw_ssg3d *g, void *stream);
+/* Shell storage of the split-field class (no counterpart in the reference,
+ * which keeps 24 full grids): floats per split component for this grid shape
+ * and split-free box; host-only, reads only nx, ny, nz, order and the axes'
+ * lo/hi.  Negative: SW_EINVAL. */
+int64_t sw_ssg3d_split_cells(const sw_ssg3d *g);
+/* The (cell, slot) pairs whose split values a split-field sweep (0 velocity,
+ * 1 stress) updates, in its launches' logical block order: cell =
+ * (k*ny + j)*nx + i, slot the cell's index in the shell storage (layout
+ * above).  Writes at most max pairs and returns their total number;
+ * host-only, for callers that read split values and for tests of the
+ * layout. */
+int64_t sw_ssg3d_split_map(const sw_ssg3d *g, int32_t sweep, int64_t *cell, int64_t *slot, int64_t max);
 /* Launch plan of the two 3-D sweeps for this descriptor (no counterpart in
  * the reference, whose loops are one Taichi task each): returns the number of
  * launches per sweep (1, or 5 when the CPML-free bulk is >= 75 % of the
@@ -324,8 +354,9 @@ int sw_mlp_loss_backward_noalloc(const float *params, const float *X, const floa
  * Streaming probe: out[o][e] = (o+1) * sum_q in[q][e] for e < n, every element
  * read or written once with 16-byte accesses, nblock blocks of 256 threads
  * in a grid-stride loop.  Instantiated mixes (nin read / nout written):
- * 10/3 (the velocity sweep's arrays), 11/6 (the stress sweep's), 1/1 (a
- * copy).  n a multiple of 4, arrays 16-byte aligned and device resident. */
+ * 10/3 (the velocity sweep's arrays), 11/6 (the stress sweep's), 16/12 and
+ * 20/21 (the two sweeps at a split-field PML shell cell), 1/1 (a copy).  n a
+ * multiple of 4, arrays 16-byte aligned and device resident. */
 int sw_stream_probe(const float *const *in, int32_t nin, float *const *out, int32_t nout, int64_t n,
                     int32_t nblock, void *stream);
 

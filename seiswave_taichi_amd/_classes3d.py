@@ -135,7 +135,8 @@ class ElasticWAVE3DBase:
         e = self._e
         if e.split is None:
             raise AttributeError("this class keeps CPML memory variables, not split fields")
-        return np.ascontiguousarray(e.split[SPLIT_NAMES.index(name)].cpu().numpy().transpose(2, 1, 0))
+        comp = e.shell.component(e.split.cpu().numpy(), SPLIT_NAMES.index(name))
+        return np.ascontiguousarray(e.shell.unpack(comp).transpose(2, 1, 0))
 
     @staticmethod
     def diff_coff(order: int):

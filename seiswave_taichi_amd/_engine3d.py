@@ -35,6 +35,81 @@ def split_free_range(ax):
     return lo, hi
 
 
+class SplitShell:
+    """Shell storage of the split-field PML (include/seiswave.h, sw_ssg3d.split):
+    each of the 24 split components keeps only the cells outside the
+    split-free box [lo, hi) of every axis, where the reference's full-grid
+    split arrays are exactly 0.  Parts: the planes outside [zlo, zhi)
+    ([wz][ny][nx]); in the other planes the rows outside [ylo, yhi)
+    ([mz][wy][nx]); in those rows the columns outside [xlo, xhi)
+    ([mz][my][wx])."""
+
+    def __init__(self, shape_xyz, ranges_xyz):
+        nx, ny, nz = (int(v) for v in shape_xyz)
+        (self.xlo, self.xhi), (self.ylo, self.yhi), (self.zlo, self.zhi) = ranges_xyz
+        self.shape = (nz, ny, nx)
+        self.wx = nx - (self.xhi - self.xlo)
+        self.wy = ny - (self.yhi - self.ylo)
+        self.wz = nz - (self.zhi - self.zlo)
+        self.my = self.yhi - self.ylo
+        self.mz = self.zhi - self.zlo
+        self.OZ = self.wz * ny * nx
+        self.OY = self.OZ + self.mz * self.wy * nx
+        self.cells = self.OY + self.mz * self.my * self.wx
+        self.floats = 1536 * ((self.cells + 63) // 64)
+
+    # component c -> (group offset in a 64-slot block, group width, first component)
+    GROUPS = [(0, 4, 0)] * 4 + [(256, 4, 4)] * 4 + [(512, 1, 8)] + [(576, 4, 9)] * 4 + [(832, 4, 13)] * 4 \
+        + [(1088, 4, 17)] * 4 + [(1344, 2, 21)] * 2 + [(1472, 1, 23)]
+
+    def index(self, c, slots):
+        """Float index in the storage of component c at the given slots."""
+        off, w, c0 = self.GROUPS[c]
+        slots = np.asarray(slots, np.int64)
+        return (slots >> 6) * 1536 + off + w * (slots & 63) + (c - c0)
+
+    def component(self, storage, c) -> np.ndarray:
+        """Component c's values at slots 0..S-1 of the storage."""
+        storage = np.asarray(storage).reshape(-1)
+        if storage.size != self.floats:
+            raise ValueError(f"{storage.size} floats, the shell storage holds {self.floats}")
+        return storage[self.index(c, np.arange(self.cells))]
+
+    def _slab(self, n, lo, hi):
+        return np.r_[0:lo, hi:n]
+
+    def unpack(self, comp) -> np.ndarray:
+        """Full [nz][ny][nx] array of one component's values at slots
+        0..S-1 (0 inside the split-free box)."""
+        nz, ny, nx = self.shape
+        comp = np.asarray(comp).reshape(-1)
+        if comp.size != self.cells:
+            raise ValueError(f"{comp.size} values, the shell holds {self.cells}")
+        full = np.zeros(self.shape, dtype=comp.dtype)
+        kz = self._slab(nz, self.zlo, self.zhi)
+        jy = self._slab(ny, self.ylo, self.yhi)
+        ix = self._slab(nx, self.xlo, self.xhi)
+        full[kz] = comp[:self.OZ].reshape(self.wz, ny, nx)
+        mid = full[self.zlo:self.zhi]
+        mid[:, jy, :] = comp[self.OZ:self.OY].reshape(self.mz, self.wy, nx)
+        inner = mid[:, self.ylo:self.yhi]
+        inner[:, :, ix] = comp[self.OY:].reshape(self.mz, self.my, self.wx)
+        return full
+
+    def slot(self, i, j, k) -> int:
+        """Index of cell (i, j, k) in a component's shell storage (ValueError
+        inside the split-free box)."""
+        nz, ny, nx = self.shape
+        sl = lambda v, lo, hi: v if v < lo else v - (hi - lo)
+        if not (self.zlo <= k < self.zhi):
+            return (sl(k, self.zlo, self.zhi) * ny + j) * nx + i
+        if not (self.ylo <= j < self.yhi):
+            return self.OZ + ((k - self.zlo) * self.wy + sl(j, self.ylo, self.yhi)) * nx + i
+        if not (self.xlo <= i < self.xhi):
+            return self.OY + ((k - self.zlo) * self.my + (j - self.ylo)) * self.wx + sl(i, self.xlo, self.xhi)
+        raise ValueError(f"cell ({i}, {j}, {k}) is in the split-free box")
+
+
 def _check_fieldtype(fieldtype):
     name = getattr(fieldtype, "__name__", None) or str(fieldtype)
     if fieldtype in (None, np.float32, torch.float32) or "f32" in name or "float32" in name:
@@ -122,7 +197,8 @@ class SSG3DEngine:
         nx, ny, nz = self.nx, self.ny, self.nz
         ax, ay, az = self.pml
         if self.cls == L.SW_CLASS_SPML:
-            self.split = torch.zeros((24, nz, ny, nx), dtype=torch.float32, device=dev)
+            self.shell = SplitShell((nx, ny, nz), [split_free_range(a) for a in (ax, ay, az)])
+            self.split = torch.zeros(self.shell.floats, dtype=torch.float32, device=dev)
             self.mem = None
         else:
             self.split = None
@@ -168,7 +244,7 @@ class SSG3DEngine:
             a.in_ = C.cast(C.c_void_p(flags.data_ptr()), L._u8p)
             a.in_h = C.cast(C.c_void_p(flags.data_ptr() + n), L._u8p)
             if self.cls == L.SW_CLASS_SPML:
-                a.lo, a.hi = split_free_range(ax)
+                a.lo, a.hi = split_free_range(ax)  # the shell's box (SplitShell)
             else:
                 a.lo, a.hi = ax.lo, ax.hi
             setattr(g, name, a)

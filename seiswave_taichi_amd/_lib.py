@@ -69,6 +69,9 @@ class SwBand(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("ox", "oy", "oz", "ex", "ey", "ez")]
 
 
+# Return types other than int.
+RESTYPES = {"sw_ssg3d_split_cells": C.c_int64, "sw_ssg3d_split_map": C.c_int64}
+
 # Every exported symbol of include/seiswave.h with its argument types.
 _P3, _P2 = C.POINTER(SwSSG3D), C.POINTER(SwSSG2D)
 _PS, _PR = C.POINTER(SwSource), C.POINTER(SwReceivers)
@@ -83,6 +86,8 @@ SIGNATURES = {
     "sw_ssg3d_velocity_spml": [_P3, C.c_void_p],
     "sw_ssg3d_stress_spml": [_P3, C.c_void_p],
     "sw_ssg3d_launch_plan": [_P3, C.POINTER(C.c_int32), C.c_int32],
+    "sw_ssg3d_split_cells": [_P3],
+    "sw_ssg3d_split_map": [_P3, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64],
     "sw_ssg3d_inject": [_P3, _PS, C.c_int32, C.c_void_p],
     "sw_ssg3d_record": [_P3, _PR, C.c_int32, C.c_void_p],
     "sw_ssg3d_run": [_P3, _PS, _PR, C.c_int32, C.c_int32, C.c_void_p],
@@ -146,7 +151,7 @@ def load(path: str | None = None):
     for name, argtypes in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = argtypes
-        fn.restype = C.c_int
+        fn.restype = RESTYPES.get(name, C.c_int)
     if path is None:
         _lib = lib
     return lib
@@ -186,3 +191,22 @@ def launch_plan(g) -> list:
     if n < 0:
         check(n, "sw_ssg3d_launch_plan")
     return list(pm[:n])
+
+
+def split_map(g, sweep=1):
+    """(cell, slot) int64 arrays of every cell whose split values the
+    split-field sweep ``sweep`` (0 velocity, 1 stress) updates for descriptor
+    ``g``: cell = (k*ny + j)*nx + i, slot its index in the shell storage.
+    Host only."""
+    import numpy as np
+    lib = load()
+    n = lib.sw_ssg3d_split_map(C.byref(g), sweep, None, None, 0)
+    if n < 0:
+        check(int(n), "sw_ssg3d_split_map")
+    cell = np.empty(n, np.int64)
+    slot = np.empty(n, np.int64)
+    p64 = C.POINTER(C.c_int64)
+    m = lib.sw_ssg3d_split_map(C.byref(g), sweep, cell.ctypes.data_as(p64), slot.ctypes.data_as(p64), n)
+    if m != n:
+        raise SeisWaveError(f"sw_ssg3d_split_map returned {m} pairs, expected {n}")
+    return cell, slot

@@ -24,6 +24,7 @@ int launch_ssg3d_stress(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_velocity_spml(const sw_ssg3d *g, hipStream_t st);
 int launch_ssg3d_stress_spml(const sw_ssg3d *g, hipStream_t st);
 int ssg3d_plan_masks(const sw_ssg3d *g, int32_t *pm, int32_t max);
+long split_shell_map(const sw_ssg3d *g, int sweep, int64_t *cell, int64_t *slot, long max);
 int launch_ssg3d_velocity_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
 int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
 int launch_ssg2d_velocity(const sw_ssg2d *g, hipStream_t st);
@@ -109,6 +110,8 @@ int check_ssg3d(const sw_ssg3d *g)
     if ((rc = check_axis(g->pz, g->nz, "z", !spml, spml)) != SW_OK) return rc;
     if (spml) {
         SW_REQUIRE(g->split != nullptr, "split-field arrays missing");
+        // the sweeps move the split values with 16-byte accesses
+        SW_REQUIRE(reinterpret_cast<uintptr_t>(g->split) % 16 == 0, "split-field storage not 16-byte aligned");
     } else {
         SW_REQUIRE(g->px.lo + g->nx - g->px.hi == 0 || g->mem_x, "x memory slab missing");
         SW_REQUIRE(g->py.lo + g->ny - g->py.hi == 0 || g->mem_y, "y memory slab missing");
@@ -465,6 +468,38 @@ int sw_ssg3d_launch_plan(const sw_ssg3d *g, int32_t *pm, int32_t max)
     if (rc) return rc;
     SW_REQUIRE(max >= 0 && (max == 0 || pm), "null mask array");
     return ssg3d_plan_masks(g, pm, max);
+}
+
+// Shell storage of the split-field class: host-only, from the grid shape and
+// the axes' lo/hi (no device pointer is read).
+static int check_split_shape(const sw_ssg3d *g)
+{
+    SW_REQUIRE(g != nullptr, "null descriptor");
+    SW_REQUIRE(g->order >= 1 && g->order <= SW_MAX_ORDER, "order %d outside 1..%d", g->order, SW_MAX_ORDER);
+    SW_REQUIRE(g->nx >= 2 * g->order + 2 && g->ny >= 2 * g->order + 2 && g->nz >= 2 * g->order + 2,
+               "grid %dx%dx%d too small for order %d", g->nx, g->ny, g->nz, g->order);
+    const sw_axis *ax[3] = {&g->px, &g->py, &g->pz};
+    const int n[3] = {g->nx, g->ny, g->nz};
+    for (int d = 0; d < 3; ++d)
+        SW_REQUIRE(ax[d]->lo >= 0 && ax[d]->lo <= ax[d]->hi && ax[d]->hi <= n[d],
+                   "axis %c: split-free range lo=%d hi=%d outside [0,%d]", "xyz"[d], ax[d]->lo, ax[d]->hi, n[d]);
+    return SW_OK;
+}
+
+int64_t sw_ssg3d_split_cells(const sw_ssg3d *g)
+{
+    int rc = check_split_shape(g);
+    if (rc) return rc;
+    return split_shell(*g).S;
+}
+
+int64_t sw_ssg3d_split_map(const sw_ssg3d *g, int32_t sweep, int64_t *cell, int64_t *slot, int64_t max)
+{
+    int rc = check_split_shape(g);
+    if (rc) return rc;
+    SW_REQUIRE(sweep == 0 || sweep == 1, "sweep %d: 0 (velocity) or 1 (stress)", sweep);
+    SW_REQUIRE(max >= 0 && (max == 0 || (cell && slot)), "null output arrays");
+    return split_shell_map(g, sweep, cell, slot, max);
 }
 
 int sw_ssg3d_inject(const sw_ssg3d *g, const sw_source *src, int32_t it, void *stream)

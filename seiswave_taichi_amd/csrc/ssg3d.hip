@@ -2,9 +2,9 @@
 //
 // Reference: src/Elastic3D_SSG_CPML.py:144-309 (update_SSG),
 // src/Elastic3D_SSG_Tensor.py:166-345 (same maths, other vz term order) and
-// src/Elastic3D_SSG_SPML.py:160-393 (split-field PML: Crank-Nicolson on the
-// damping d of each derivative direction inside the PML, plain updates with
-// 1/rho and point lambda, mu elsewhere).
+// src/Elastic3D_SSG_SPML.py:160-393 (split-field PML: plain updates with
+// 1/rho and point lambda, mu in its split-free interior; the PML shell runs
+// the kernels of ssg3d_split.hip).
 //
 // Execution: each thread owns one (i, j) column and marches along z over a
 // chunk of planes; a block takes 256 consecutive columns of a box in x-fastest
@@ -36,33 +36,9 @@
 #ifndef SW_PREFETCH
 #define SW_PREFETCH 1
 #endif
-// SW_SPLIT_PREFETCH=1 also fetches the velocity split-field values of plane
-// k+1 with the plane (split-field PML class; 4 % on that sweep); 0 loads them
-// when plane k is computed.  The stress sweep loads its 15 when computing: a
-// prefetch there measured no gain.
-#ifndef SW_SPLIT_PREFETCH
-#define SW_SPLIT_PREFETCH 1
-#endif
-
 namespace sw {
 
 namespace {
-
-// A launch covers one box of cells: columns [i0, i0+w) x [j0, j0+h) over
-// planes [k0, k1).  The box is cut into blocks of kThreads consecutive
-// columns (x fastest, so no lane idles on a ragged x edge and a wave reads at
-// most two row segments per tap) times chunks of kZChunk planes.
-constexpr int kThreads = 256;
-
-// Blocks run in groups of kGroup column blocks (8,192 columns) that march
-// through all z chunks before the next group starts.  With 4-plane chunks a
-// chunk's window planes are the ones the same group read one chunk earlier,
-// still in the XCD's L2, while a group is wide enough that its y halo rows
-// are a small part of its reads.  Measured on MI355X, ms per step with groups
-// of 4 / 8 / 16 / 24 / 32 / 48 / 64 / 96 column blocks: 200^3 0.327 / 0.322 /
-// 0.322 / 0.313 / 0.305 / 0.308 / 0.305 / 0.307, against 0.318 in
-// plane-by-plane order; 600x600x400 4.77 (32), 4.73 (64), 4.91 plane by plane.
-constexpr int kGroup = 32;
 
 struct Box {
     int i0, j0, w, h, k0, k1, ncb;  // ncb: column blocks of the box
@@ -102,55 +78,6 @@ __device__ __forceinline__ Cell box_cell(const Geo &geo)
 }
 
 
-// Split-field PML of one column (src/Elastic3D_SSG_SPML.py): in-PML flags of
-// the integer and half points on x and y (only axes the kernel instantiates;
-// the plan guarantees the others are 0) and the damping there.
-struct SpCol {
-    bool xi, xh, yi, yh;
-    float dxi, dxh, dyi, dyh;
-};
-
-template <int PM>
-__device__ __forceinline__ SpCol spml_col(const sw_ssg3d &g, int i, int j)
-{
-    SpCol c;
-    c.xi = (PM & 1) ? g.px.in[i] != 0 : false;
-    c.xh = (PM & 1) ? g.px.in_h[i] != 0 : false;
-    c.yi = (PM & 2) ? g.py.in[j] != 0 : false;
-    c.yh = (PM & 2) ? g.py.in_h[j] != 0 : false;
-    c.dxi = g.px.d[i];
-    c.dxh = g.px.d_h[i];
-    c.dyi = g.py.d[j];
-    c.dyh = g.py.d_h[j];
-    return c;
-}
-
-// The split values q[t] = split[first + t][cell] (t < M) of a cell whose
-// integer or half point lies in the PML on some axis at plane k; the others
-// are not loaded (their plain update needs none).
-template <int M, int PM>
-__device__ __forceinline__ void load_split(float *q, const sw_ssg3d &g, const SpCol &sp, long cell, int k, int first)
-{
-    const bool zi = (PM & 4) ? g.pz.in[k] != 0 : false;
-    const bool zh = (PM & 4) ? g.pz.in_h[k] != 0 : false;
-    if (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh) {
-        const long NC = (long)g.nx * g.ny * g.nz;
-        const float *qp = g.split + cell;
-#pragma unroll
-        for (int t = 0; t < M; ++t) q[t] = qp[(long)(first + t) * NC];
-    }
-}
-
-// Crank-Nicolson step of one split component: q = (n q + incr) / d
-// (src/Elastic3D_SSG_SPML.py:199-204 and alike); q0 is the value at dst.
-__device__ __forceinline__ float cn_at(float q0, float *dst, float n, float d, float incr)
-{
-    const float v = (n * q0 + incr) / d;
-    *dst = v;
-    return v;
-}
-
-
 // LARGE: arrays above 4 GiB; the block rebases its full-grid pointers to its
 // first plane (otherwise kb is 0 and the rebasing compiles away, which keeps
 // the scalar registers of the common case free).
@@ -170,13 +97,10 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
     const bool vx_col = i < nx - N - 1 && j >= N + 1;
     const bool vy_col = j < ny - N - 1 && i >= N + 1;
     const bool vz_col = i >= N + 1 && j >= N + 1;
-    // CPML axes of this instance (none for the split-field class)
+    // CPML axes of this instance (none for the split-field class, whose
+    // kernels here run only its split-free interior)
     constexpr int CPM = CLS == SW_CLASS_SPML ? 0 : PM;
-    constexpr bool SPLIT = CLS == SW_CLASS_SPML && PM != 0;
     const Slab sl = slab_setup<CPM>(g, i, j);
-    SpCol sp{};
-    if (SPLIT) sp = spml_col<PM>(g, i, j);
-    const long NC = (long)nx * ny * nz;
     const int kb = LARGE ? cl.k0 - N : 0;  // base plane of this block's offsets
     const Res R = make_res(g, kb);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
@@ -192,36 +116,15 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
         wzz[t] = ld(R.szz, pc + (uint32_t)(k0 - N + 1 + t - kb) * SZ);
     }
     VPlane<N> P;
-    const long cell0 = (long)j * nx + i, plane = (long)nx * ny;
 #if SW_PREFETCH
     load_vplane<N, CPM>(P, g, R, pc, k0, kb, SY, SZ, sl);
-    if (SW_SPLIT_PREFETCH && SPLIT) load_split<9, PM>(P.q, g, sp, cell0 + k0 * plane, k0, 0);
 #endif
     for (int k = k0; k < cl.k1; ++k) {
 #if SW_PREFETCH
         VPlane<N> Q;
-        if (k + 1 < cl.k1) {
-            load_vplane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
-            if (SW_SPLIT_PREFETCH && SPLIT) load_split<9, PM>(Q.q, g, sp, cell0 + (k + 1) * plane, k + 1, 0);
-        }
+        if (k + 1 < cl.k1) load_vplane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
 #else
         load_vplane<N, CPM>(P, g, R, pc, k, kb, SY, SZ, sl);
-#endif
-        // split-field PML: flags and damping of this plane, split arrays at (i, j, k)
-        const bool zi = SPLIT && (PM & 4) ? g.pz.in[k] != 0 : false;
-        const bool zh = SPLIT && (PM & 4) ? g.pz.in_h[k] != 0 : false;
-        const float dzi = SPLIT ? g.pz.d[k] : 0.f, dzh = SPLIT ? g.pz.d_h[k] : 0.f;
-        float *qp = SPLIT ? g.split + ((long)k * ny + j) * nx + i : nullptr;
-        const float hdt = 0.5f * dt;
-        // all split values of this cell first, so their loads overlap
-        float qv[9];
-#if SW_PREFETCH && SW_SPLIT_PREFETCH
-        if (SPLIT) copy_n<9>(qv, P.q);
-#else
-        if (SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh)) {
-#pragma unroll
-            for (int t = 0; t < 9; ++t) qv[t] = qp[(long)t * NC];
-        }
 #endif
         const uint32_t p = pc + (uint32_t)(k - kb) * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
@@ -246,17 +149,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
                 dsxz = cpml_step(dsxz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 0 * sl.nzs + oz, P.mz[0]);
             }
-            if (CLS == SW_CLASS_SPML) {  // src/Elastic3D_SSG_SPML.py:182-205: x+y+z, times 1/rho
-                const float rho_over = 1.f / P.rho[0];
-                if (SPLIT && (sp.xh | sp.yi | zi)) {
-                    const float a = cn_at(qv[0], qp + 0 * NC, 1.f - hdt * sp.dxh, 1.f + hdt * sp.dxh, dsxx * dt * rho_over);
-                    const float b = cn_at(qv[1], qp + 1 * NC, 1.f - hdt * sp.dyi, 1.f + hdt * sp.dyi, dsxy * dt * rho_over);
-                    const float e = cn_at(qv[2], qp + 2 * NC, 1.f - hdt * dzi, 1.f + hdt * dzi, dsxz * dt * rho_over);
-                    st(R.vx, p, a + b + e);
-                } else {
-                    st(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * rho_over);
-                }
-            } else
+            if (CLS == SW_CLASS_SPML)  // src/Elastic3D_SSG_SPML.py:203: x+y+z, times 1/rho
+                st(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * (1.f / P.rho[0]));
+            else
                 st(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
         }
         // vy at (i, j+1/2, k): :180-195
@@ -277,17 +172,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
                 dsyz = cpml_step(dsyz, P.mz[1], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 1 * sl.nzs + oz, P.mz[1]);
             }
-            if (CLS == SW_CLASS_SPML) {  // :209-233
-                const float rho_over = 1.f / rho_half_y;
-                if (SPLIT && (sp.xi | sp.yh | zi)) {
-                    const float a = cn_at(qv[3], qp + 3 * NC, 1.f - hdt * sp.dxi, 1.f + hdt * sp.dxi, dt * dsxy * rho_over);
-                    const float b = cn_at(qv[4], qp + 4 * NC, 1.f - hdt * sp.dyh, 1.f + hdt * sp.dyh, dt * dsyy * rho_over);
-                    const float e = cn_at(qv[5], qp + 5 * NC, 1.f - hdt * dzi, 1.f + hdt * dzi, dt * dsyz * rho_over);
-                    st(R.vy, p, a + b + e);
-                } else {
-                    st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * rho_over);
-                }
-            } else
+            if (CLS == SW_CLASS_SPML)  // :231
+                st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * (1.f / rho_half_y));
+            else
                 st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
         }
         // vz at (i, j, k+1/2): :197-212 (Tensor :228-243)
@@ -308,17 +195,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
                 dszz = cpml_step(dszz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 2 * sl.nzs + oz, P.mz[2]);
             }
-            if (CLS == SW_CLASS_SPML) {  // :235-259
-                const float rho_over = 1.f / rho_avg;
-                if (SPLIT && (sp.xi | sp.yi | zh)) {
-                    const float a = cn_at(qv[6], qp + 6 * NC, 1.f - hdt * sp.dxi, 1.f + hdt * sp.dxi, dt * dsxz * rho_over);
-                    const float b = cn_at(qv[7], qp + 7 * NC, 1.f - hdt * sp.dyi, 1.f + hdt * sp.dyi, dt * dsyz * rho_over);
-                    const float e = cn_at(qv[8], qp + 8 * NC, 1.f - hdt * dzh, 1.f + hdt * dzh, dt * dszz * rho_over);
-                    st(R.vz, p, a + b + e);
-                } else {
-                    st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * rho_over);
-                }
-            } else if (CLS == SW_CLASS_TENSOR)
+            if (CLS == SW_CLASS_SPML)  // :257
+                st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * (1.f / rho_avg));
+            else if (CLS == SW_CLASS_TENSOR)
                 st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt / rho_avg);
             else
                 st(R.vz, p, P.v[2] + (dszz + dsxz + dsyz) * dt / rho_avg);
@@ -394,20 +273,6 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
     }
 }
 
-// Velocity receivers of this step, gathered by the nb blocks appended to the
-// first stress launch: data[(it*ncomp + comp)*nrec + r].
-__device__ void gather3d(const sw_ssg3d &g, const Fold3D &fo, int b, int nb)
-{
-    const sw_receivers &r = fo.rec;
-    const int n = r.nrec * r.ncomp;
-    float *out = r.data + (long)fo.it_rec * n;
-    for (int t = b * kThreads + (int)threadIdx.x; t < n; t += nb * kThreads) {
-        const int comp = t / r.nrec, q = t - comp * r.nrec;
-        const int code = r.comp[comp];
-        out[t] = (code == 0 ? g.vx : code == 1 ? g.vy : g.vz)[r.idx[q]];
-    }
-}
-
 template <int N, int CLS, int PM, bool LARGE>
 __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D fo)
 {
@@ -430,11 +295,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
     const bool sxz_col = i < nx - N - 1;
     const bool syz_col = j < ny - N - 1;
     constexpr int CPM = CLS == SW_CLASS_SPML ? 0 : PM;
-    constexpr bool SPLIT = CLS == SW_CLASS_SPML && PM != 0;
     const Slab sl = slab_setup<CPM>(g, i, j);
-    SpCol sp{};
-    if (SPLIT) sp = spml_col<PM>(g, i, j);
-    const long NC = (long)nx * ny * nz;
     const int kb = LARGE ? cl.k0 - N : 0;  // base plane of this block's offsets
     const Res R = make_res(g, kb);
     const Coef cx = coef_of(g.px, i, sl.xs >= 0);
@@ -460,19 +321,6 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
 #else
         load_splane<N, CPM>(P, g, R, pc, k, kb, SY, SZ, sl);
 #endif
-        const bool zi = SPLIT && (PM & 4) ? g.pz.in[k] != 0 : false;
-        const bool zh = SPLIT && (PM & 4) ? g.pz.in_h[k] != 0 : false;
-        const float dzi = SPLIT ? g.pz.d[k] : 0.f, dzh = SPLIT ? g.pz.d_h[k] : 0.f;
-        float *qp = SPLIT ? g.split + ((long)k * ny + j) * nx + i : nullptr;
-        const float hdt = 0.5f * dt;
-        // all split values of this cell first (components 9..23), so their
-        // loads overlap
-        float qv[15];
-        const bool any_split = SPLIT && (sp.xi | sp.xh | sp.yi | sp.yh | zi | zh);
-        if (any_split) {
-#pragma unroll
-            for (int t = 0; t < 15; ++t) qv[t] = qp[(long)(9 + t) * NC];
-        }
         const uint32_t p = pc + (uint32_t)(k - kb) * SZ;
         const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
         const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
@@ -501,26 +349,9 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvzz = cpml_step(dvzz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 3 * sl.nzs + oz, P.mz[0]);
             }
-            if (SPLIT && (sp.xi | sp.yi | zi)) {  // src/Elastic3D_SSG_SPML.py:296-333
-                const float xn = 1.f - hdt * sp.dxi, yn = 1.f - hdt * sp.dyi, zn = 1.f - hdt * dzi;
-                const float xd = 1.f + hdt * sp.dxi, yd = 1.f + hdt * sp.dyi, zd = 1.f + hdt * dzi;
-                const float a0 = cn_at(qv[0], qp + 9 * NC, xn, xd, dt * lpm * dvxx);
-                const float a1 = cn_at(qv[1], qp + 10 * NC, yn, yd, dt * lam_h * dvyy);
-                const float a2 = cn_at(qv[2], qp + 11 * NC, zn, zd, dt * lam_h * dvzz);
-                const float b0 = cn_at(qv[3], qp + 12 * NC, xn, xd, dt * lam_h * dvxx);
-                const float b1 = cn_at(qv[4], qp + 13 * NC, yn, yd, dt * lpm * dvyy);
-                const float b2 = cn_at(qv[5], qp + 14 * NC, zn, zd, dt * lam_h * dvzz);
-                const float c0 = cn_at(qv[12], qp + 21 * NC, xn, xd, dt * lam_h * dvxx);
-                const float c1 = cn_at(qv[13], qp + 22 * NC, yn, yd, dt * lam_h * dvyy);
-                const float c2 = cn_at(qv[14], qp + 23 * NC, zn, zd, dt * lpm * dvzz);
-                st(R.sxx, p, a0 + a1 + a2);
-                st(R.syy, p, b0 + b1 + b2);
-                st(R.szz, p, c0 + c1 + c2);
-            } else {
-                st(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
-                st(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
-                st(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
-            }
+            st(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
+            st(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
+            st(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
         }
         // sxy at (i+1/2, j+1/2, k): :267-279
         if (sxy_col) {
@@ -535,13 +366,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvxy = cpml_step(dvxy, P.my[1], cy.ah, cy.bh, cy.kh);
                 st(R.my, 4 * sl.nys + oy, P.my[1]);
             }
-            if (SPLIT && (sp.xh | sp.yh | zi)) {  // :334-350
-                const float a = cn_at(qv[6], qp + 15 * NC, 1.f - hdt * sp.dxh, 1.f + hdt * sp.dxh, dt * mu_a * dvyx);
-                const float b = cn_at(qv[7], qp + 16 * NC, 1.f - hdt * sp.dyh, 1.f + hdt * sp.dyh, dt * mu_a * dvxy);
-                st(R.sxy, p, a + b);
-            } else {
-                st(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
-            }
+            st(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
         }
         // sxz at (i+1/2, j, k+1/2): :281-293
         if (sxz_col && k < nz - N - 1) {
@@ -556,13 +381,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvxz = cpml_step(dvxz, P.mz[1], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 4 * sl.nzs + oz, P.mz[1]);
             }
-            if (SPLIT && (sp.xh | sp.yi | zh)) {  // :352-371
-                const float a = cn_at(qv[8], qp + 17 * NC, 1.f - hdt * sp.dxh, 1.f + hdt * sp.dxh, dt * mu_a * dvzx);
-                const float b = cn_at(qv[9], qp + 18 * NC, 1.f - hdt * dzh, 1.f + hdt * dzh, dt * mu_a * dvxz);
-                st(R.sxz, p, a + b);
-            } else {
-                st(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
-            }
+            st(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
         }
         // syz at (i, j+1/2, k+1/2): :295-307
         if (syz_col && k < nz - N - 1) {
@@ -577,13 +396,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvyz = cpml_step(dvyz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 5 * sl.nzs + oz, P.mz[2]);
             }
-            if (SPLIT && (sp.xi | sp.yh | zh)) {  // :373-391
-                const float a = cn_at(qv[10], qp + 19 * NC, 1.f - hdt * sp.dyh, 1.f + hdt * sp.dyh, dt * mu_a * dvzy);
-                const float b = cn_at(qv[11], qp + 20 * NC, 1.f - hdt * dzh, 1.f + hdt * dzh, dt * mu_a * dvyz);
-                st(R.syz, p, a + b);
-            } else {
-                st(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
-            }
+            st(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
         }
         win_shift<N>(wvz);
         win_shift<N>(wvx);
@@ -664,7 +477,9 @@ void launch_velocity(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStre
 {
     for (int t = 0; t < p.n; ++t) {
         const dim3 grd(p.g[t].nblock), blk(kThreads);
-        if (p.pm[t] == 1)
+        if (p.pm[t] == 0)
+            hipLaunchKernelGGL((k_velocity<N, CLS, 0, LARGE>), grd, blk, 0, st, g, p.g[t], fo);
+        else if (p.pm[t] == 1)
             hipLaunchKernelGGL((k_velocity<N, CLS, 1, LARGE>), grd, blk, 0, st, g, p.g[t], fo);
         else if (p.pm[t] == 3)
             hipLaunchKernelGGL((k_velocity<N, CLS, 3, LARGE>), grd, blk, 0, st, g, p.g[t], fo);
@@ -684,7 +499,9 @@ void launch_stress(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream
     for (int t = 0; t < p.n; ++t) {
         const Fold3D &f = t == 0 ? fo : rest;
         const dim3 grd(p.g[t].nblock + (f.record ? f.gather : 0)), blk(kThreads);
-        if (p.pm[t] == 1)
+        if (p.pm[t] == 0)
+            hipLaunchKernelGGL((k_stress<N, CLS, 0, LARGE>), grd, blk, 0, st, g, p.g[t], f);
+        else if (p.pm[t] == 1)
             hipLaunchKernelGGL((k_stress<N, CLS, 1, LARGE>), grd, blk, 0, st, g, p.g[t], f);
         else if (p.pm[t] == 3)
             hipLaunchKernelGGL((k_stress<N, CLS, 3, LARGE>), grd, blk, 0, st, g, p.g[t], f);
@@ -697,6 +514,17 @@ void launch_stress(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream
 
 // Arrays above 4 GiB take the rebasing kernels.
 inline bool large_grid(const sw_ssg3d &g) { return 4L * g.nx * g.ny * g.nz > (1L << 32); }
+
+// The split-free interior of the split-field class in the stress sweep: one
+// PM = 0 launch (plain updates, no split values); the shell around it is
+// launched by launch_split_shell_stress (ssg3d_split.hip).
+Plan make_interior_plan(const sw_ssg3d &g, int N)
+{
+    Plan p;
+    const SplitBox b = split_box(g, N);
+    add_box(p, 0, b.xl, b.xh, b.yl, b.yh, b.zl, b.zh);
+    return p;
+}
 
 template <int N, int CLS>
 void launch_velocity_any(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipStream_t st)
@@ -718,35 +546,49 @@ void launch_stress_any(const sw_ssg3d &g, const Plan &p, const Fold3D &fo, hipSt
 
 int launch_ssg3d_velocity_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st)
 {
+    // split-field class: its slabs and full-width middle rows (ssg3d_split.hip)
+    if (g->cls == SW_CLASS_SPML) return launch_split_shell_velocity(g, fo, st);
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
         if (g->cls == SW_CLASS_TENSOR)
             launch_velocity_any<N, SW_CLASS_TENSOR>(*g, p, fo, st);
-        else if (g->cls == SW_CLASS_SPML)
-            launch_velocity_any<N, SW_CLASS_SPML>(*g, p, fo, st);
         else
             launch_velocity_any<N, SW_CLASS_CPML>(*g, p, fo, st);
     });
-    return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d velocity (split-field PML)" : "ssg3d velocity");
+    return launch_status("ssg3d velocity");
 }
 
-// Split-field PML class (src/Elastic3D_SSG_SPML.py:160-393): the same plan,
-// the axes' lo/hi bounding their split-free ranges.
+// Split-field PML class (src/Elastic3D_SSG_SPML.py:160-393): the shell launch
+// (which carries the folded receiver gather) and the interior launch.
 int launch_ssg3d_stress_fold(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st)
 {
+    if (g->cls == SW_CLASS_SPML) {
+        int rc = launch_split_shell_stress(g, fo, st);
+        if (rc) return rc;
+        Fold3D rest = fo;
+        rest.record = 0;
+        rest.gather = 0;
+        SW_DISPATCH_ORDER(g->order, {
+            const Plan p = make_interior_plan(*g, N);
+            launch_stress_any<N, SW_CLASS_SPML>(*g, p, rest, st);
+        });
+        return launch_status("ssg3d stress (split-field PML interior)");
+    }
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
-        if (g->cls == SW_CLASS_SPML)
-            launch_stress_any<N, SW_CLASS_SPML>(*g, p, fo, st);
-        else
-            launch_stress_any<N, SW_CLASS_CPML>(*g, p, fo, st);
+        launch_stress_any<N, SW_CLASS_CPML>(*g, p, fo, st);
     });
-    return launch_status(g->cls == SW_CLASS_SPML ? "ssg3d stress (split-field PML)" : "ssg3d stress");
+    return launch_status("ssg3d stress");
 }
 
 int ssg3d_plan_masks(const sw_ssg3d *g, int32_t *pm, int32_t max)
 {
     int n = 0;
+    if (g->cls == SW_CLASS_SPML) {  // shell (flags of all three axes), interior
+        const int32_t m[2] = {7, 0};
+        for (int t = 0; t < 2 && t < max; ++t) pm[t] = m[t];
+        return 2;
+    }
     SW_DISPATCH_ORDER(g->order, {
         const Plan p = make_plan(*g, N);
         n = p.n;

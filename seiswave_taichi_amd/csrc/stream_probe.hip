@@ -11,7 +11,7 @@ namespace sw {
 
 namespace {
 
-constexpr int kMaxIn = 16, kMaxOut = 8;
+constexpr int kMaxIn = 20, kMaxOut = 21;
 
 struct StreamArgs {
     const float *in[kMaxIn];
@@ -49,15 +49,23 @@ int launch_stream_probe(const float *const *in, int nin, float *const *out, int 
     for (int o = 0; o < nout; ++o) a.out[o] = out[o];
     const long n4 = n / 4;
     // the mixes of the two sweeps (velocity: 10 read, 3 written; stress: 11
-    // read, 6 written) and of a copy
+    // read, 6 written), of their split-field shell cells (velocity: 6
+    // stresses, rho and 9 split values read, 3 velocities and the 9 split
+    // values written; stress: 3 velocities, lambda, mu and 15 split values
+    // read, 6 stresses and the 15 split values written) and of a copy
     if (nin == 10 && nout == 3)
         launch<10, 3>(a, n4, nblock, st);
     else if (nin == 11 && nout == 6)
         launch<11, 6>(a, n4, nblock, st);
+    else if (nin == 16 && nout == 12)
+        launch<16, 12>(a, n4, nblock, st);
+    else if (nin == 20 && nout == 21)
+        launch<20, 21>(a, n4, nblock, st);
     else if (nin == 1 && nout == 1)
         launch<1, 1>(a, n4, nblock, st);
     else {
-        set_error("stream probe: mix %d read / %d written not instantiated (10/3, 11/6, 1/1)", nin, nout);
+        set_error("stream probe: mix %d read / %d written not instantiated (10/3, 11/6, 16/12, 20/21, 1/1)", nin,
+                  nout);
         return SW_EINVAL;
     }
     return launch_status("stream probe");

@@ -105,6 +105,73 @@ __device__ __forceinline__ int slab_index(int i, int lo, int hi)
     return i < lo ? i : (i >= hi ? lo + (i - hi) : -1);
 }
 
+// Shell storage of the split-field PML.  The reference keeps 24 full-grid
+// split arrays but reads and writes them only where a component's in-PML
+// test holds, i.e. outside the split-free box [xlo,xhi) x [ylo,yhi) x
+// [zlo,zhi) (the axes' lo/hi).  Each component here holds only the S cells
+// of the shell, in three parts:
+//   z part  [wz][ny][nx]  planes k < zlo or k >= zhi (plane index zs(k));
+//   y part  [mz][wy][nx]  planes zlo <= k < zhi, rows j < ylo or j >= yhi;
+//   x part  [mz][my][wx]  those planes, rows ylo <= j < yhi, columns outside
+//                         [xlo, xhi);
+// with w_d = n_d - (hi_d - lo_d), m_d = hi_d - lo_d and slab index
+// s(i) = i < lo ? i : i - (hi - lo).  An axis with lo = hi has no split-free
+// range: its slab is the whole axis.  Cells are numbered ("slots") through
+// the three parts in that order.  The 24 components of 64 consecutive slots
+// share one block of 1536 floats, in groups that a sweep thread moves with
+// one 16-, 8- or 4-byte access each (float offsets in the block, lane
+// l = slot mod 64):
+//   comps 0-3 at 0 + 4l, 4-7 at 256 + 4l, 8 at 512 + l        (velocity sweep)
+//   comps 9-12 at 576 + 4l, 13-16 at 832 + 4l, 17-20 at 1088 + 4l,
+//   21-22 at 1344 + 2l, 23 at 1472 + l                          (stress sweep)
+constexpr int kSplitBlock = 64, kSplitBlockFloats = 24 * kSplitBlock;
+
+struct SplitShell {
+    int xlo, xhi, ylo, yhi, zlo, zhi;
+    int wx, wy, wz, my, mz;
+    long S, OZ, OY;  // cells per component; first cell of the y and x parts
+};
+
+inline SplitShell split_shell(const sw_ssg3d &g)
+{
+    SplitShell s;
+    s.xlo = g.px.lo; s.xhi = g.px.hi;
+    s.ylo = g.py.lo; s.yhi = g.py.hi;
+    s.zlo = g.pz.lo; s.zhi = g.pz.hi;
+    s.wx = g.nx - (s.xhi - s.xlo);
+    s.wy = g.ny - (s.yhi - s.ylo);
+    s.wz = g.nz - (s.zhi - s.zlo);
+    s.my = s.yhi - s.ylo;
+    s.mz = s.zhi - s.zlo;
+    s.OZ = (long)s.wz * g.ny * g.nx;
+    s.OY = s.OZ + (long)s.mz * s.wy * g.nx;
+    s.S = s.OY + (long)s.mz * s.my * s.wx;
+    return s;
+}
+
+// Updated cells [x0,x1) x [y0,y1) x [z0,z1) (= [N, n-N) per axis) and the
+// split-free box clamped into them, [xl,xh) x [yl,yh) x [zl,zh) (empty on an
+// axis without a split-free range).  The shell launch covers the updated
+// cells outside that box: planes [z0,zl) and [zh,z1); in [zl,zh) the rows
+// outside [yl,yh); in those rows the columns outside [xl,xh).
+struct SplitBox {
+    int x0, x1, y0, y1, z0, z1;
+    int xl, xh, yl, yh, zl, zh;
+};
+
+inline SplitBox split_box(const sw_ssg3d &g, int N)
+{
+    auto clamp = [](int v, int a, int b) { return v < a ? a : (v > b ? b : v); };
+    SplitBox b;
+    b.x0 = N; b.x1 = g.nx - N;
+    b.y0 = N; b.y1 = g.ny - N;
+    b.z0 = N; b.z1 = g.nz - N;
+    b.xl = clamp(g.px.lo, b.x0, b.x1); b.xh = clamp(g.px.hi, b.xl, b.x1);
+    b.yl = clamp(g.py.lo, b.y0, b.y1); b.yh = clamp(g.py.hi, b.yl, b.y1);
+    b.zl = clamp(g.pz.lo, b.z0, b.z1); b.zh = clamp(g.pz.hi, b.zl, b.z1);
+    return b;
+}
+
 // ---------------------------------------------------------------------------
 // Staggered first derivatives (src/Differential3D.py:7-44): the accumulator
 // starts at 0 and adds c[id]*(difference) in id order, so the rounding matches
@@ -195,6 +262,12 @@ struct Fold3D {
     int record, it_rec, gather;
     sw_receivers rec;
 };
+
+// Split-field PML shell of the 3-D sweeps (ssg3d_split.hip): one launch per
+// sweep over the updated cells outside the split-free box (the stress launch
+// carries fo's receiver gather).
+int launch_split_shell_velocity(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
+int launch_split_shell_stress(const sw_ssg3d *g, const Fold3D &fo, hipStream_t st);
 
 // Point sources of the 2-D step at cell p, shared by the injection kernel and
 // the sweeps that fold the injection in (the same expressions, so the same

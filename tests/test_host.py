@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "seiswave.h")
 
 def header_functions():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^int\s+(sw_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^int(?:64_t)?\s+(sw_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_library_exports_every_header_symbol():
@@ -387,3 +387,73 @@ def test_src_package_at_repository_root():
         "Elastic2D_CPML", "Elastic2D_sparse", "Elastic3D_SSG_CPML", "Elastic3D_SSG_SPML",
         "Elastic3D_SSG_Tensor", "wigb")]
 
+
+
+@pytest.mark.parametrize("case", [
+    # (shape, order, (xlo, xhi), (ylo, yhi), (zlo, zhi))
+    ((40, 36, 44), 2, (9, 31), (8, 28), (10, 34)),
+    ((30, 30, 30), 1, (0, 24), (5, 30), (6, 23)),      # x-lo and y-hi faces without PML
+    ((33, 28, 31), 4, (0, 0), (9, 19), (10, 21)),      # no split-free x range: every row is shell
+    ((26, 29, 24), 3, (8, 18), (9, 20), (0, 0)),       # no split-free z range: every plane is shell
+    ((24, 24, 24), 2, (0, 0), (0, 0), (0, 0)),         # all shell
+    ((25, 27, 23), 2, (0, 25), (0, 27), (0, 23)),      # no PML at all: empty shell
+])
+def test_split_shell_layout_matches_the_sweeps(case):
+    """The split-field class keeps its 24 split components only on the shell
+    around the split-free box.  The (cell, slot) pairs the shell launch
+    computes (the kernels' own index function, run on the host) cover every
+    updated cell outside the box exactly once, with distinct slots inside
+    each component's storage, at the slots the host-side unpacking reads."""
+    from seiswave_taichi_amd._engine3d import SplitShell
+    (nx, ny, nz), order, rx, ry, rz = case
+    g = _fake_3d(order=order)
+    g.cls = L.SW_CLASS_SPML
+    g.nx, g.ny, g.nz = nx, ny, nz
+    for ax, (lo, hi) in zip((g.px, g.py, g.pz), (rx, ry, rz)):
+        ax.lo, ax.hi = lo, hi
+    sh = SplitShell((nx, ny, nz), (rx, ry, rz))
+    assert L.load().sw_ssg3d_split_cells(g) == sh.cells
+    cell, slot = L.split_map(g, sweep=1)
+    # the velocity sweep's launches update the same cells at the same slots
+    cv, sv = L.split_map(g, sweep=0)
+    by_cell = np.argsort(cv)
+    np.testing.assert_array_equal(cv[by_cell], np.sort(cell))
+    np.testing.assert_array_equal(sv[by_cell], slot[np.argsort(cell)])
+    N = order
+    k, r = np.divmod(cell, nx * ny)
+    j, i = np.divmod(r, nx)
+    upd = (slice(N, nz - N), slice(N, ny - N), slice(N, nx - N))
+    inside = np.zeros((nz, ny, nx), bool)
+    inside[rz[0]:rz[1], ry[0]:ry[1], rx[0]:rx[1]] = True
+    want = np.zeros((nz, ny, nx), bool)
+    want[upd] = ~inside[upd]
+    got = np.zeros((nz, ny, nx), np.int64)
+    np.add.at(got, (k, j, i), 1)
+    np.testing.assert_array_equal(got, want.astype(np.int64))
+    assert np.unique(slot).size == slot.size
+    assert slot.size == 0 or (slot.min() >= 0 and slot.max() < sh.cells)
+    expect = np.array([sh.slot(a, b, c) for a, b, c in zip(i, j, k)], np.int64)
+    np.testing.assert_array_equal(slot, expect)
+    # unpacking puts slot s of a component at its cell
+    comp = np.zeros(sh.cells, np.float64)
+    comp[slot] = cell + 1.0
+    full = sh.unpack(comp).reshape(-1)
+    np.testing.assert_array_equal(full[cell], cell + 1.0)
+    assert np.count_nonzero(full) == cell.size
+    # the 24 components of the blocked storage occupy distinct floats, all of them
+    idx = np.concatenate([sh.index(c, np.arange(sh.cells)) for c in range(24)])
+    assert np.unique(idx).size == idx.size == 24 * sh.cells
+    assert idx.size == 0 or (idx.min() >= 0 and idx.max() < sh.floats)
+    if sh.cells % 64 == 0:
+        assert idx.size == sh.floats
+    storage = np.zeros(sh.floats)
+    storage[sh.index(13, slot)] = cell + 2.0
+    np.testing.assert_array_equal(sh.component(storage, 13)[slot], cell + 2.0)
+
+
+def test_split_shell_refuses_bad_ranges():
+    g = _fake_3d()
+    g.cls = L.SW_CLASS_SPML
+    g.px.lo, g.px.hi = 10, 5
+    assert L.load().sw_ssg3d_split_cells(g) == -1
+    assert "split-free range" in L.last_error()
