@@ -381,6 +381,7 @@ def learned_workload(args):
     r = LA.reflection_study(net, n_int=155, steps=300, seed=0, f=10.0, model=model, src=(77, 77, 77),
                             timing_steps=args.steps)
     ceiling = LA.linear_ceiling()
+    history = LA.history_study()
     # the plain CPML at about the wide layer's reflection: 16 cells, R = 1e-3 (192^3 grid)
     r16 = LA.reflection_study(net, n_int=155, steps=300, seed=0, f=10.0, model=model, src=(77, 77, 77),
                               thin=16, thin_rcoef=1e-3, timing_steps=args.steps)
@@ -405,7 +406,10 @@ def learned_workload(args):
            "linear_map_removes_of_one_step_error": {
                "in_sample_upper_bound": {k: round(v[0], 4) for k, v in ceiling.items()},
                "held_out": {k: round(v[1], 4) for k, v in ceiling.items()}},
-           "held_out_one_step_residual": [round(h[-1][2], 4) for h in hists.values()]}
+           "held_out_one_step_residual": [round(h[-1][2], 4) for h in hists.values()],
+           "linear_map_with_4_steps_of_band_history": {
+               "held_out_removes_of_one_step_error": {k: round(v, 5) for k, v in history["held_out"].items()},
+               "closed_loop_reflection_vs_thin": {k: float(f"{v:.4g}") for k, v in history["closed_loop"].items()}}}
     print(json.dumps(out), flush=True)
 
 

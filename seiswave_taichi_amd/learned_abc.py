@@ -469,7 +469,7 @@ def tune_output_scale(net: BandNet, alphas=(1.0, 0.5, 0.25, 0.1, 0.05), n_int=28
 
 
 def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3, f=15.0, thin=4, wide=20,
-                     model=None, src=None, every=5, timing_steps=0, thin_rcoef=None):
+                     model=None, src=None, every=5, timing_steps=0, thin_rcoef=None, runner=None):
     """Reflected energy of thin CPML (with and without the network) and of the
     wide CPML, against a padded reference whose own boundary reflections
     cannot reach the interior within ``steps``.
@@ -479,7 +479,8 @@ def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3,
     steps and over the interior box of (v - v_ref)^2 for the three velocity
     components.  ``timing_steps`` > 0 also times the thin (alone and with
     the network) and the wide runs (ms per step, HIP events).  ``thin_rcoef``
-    overrides the thin layer's target reflection coefficient."""
+    overrides the thin layer's target reflection coefficient; ``runner(sim)``,
+    if given, replaces LearnedRun(sim, net) for the corrected thin run."""
     rng = np.random.default_rng(seed)
     if model is None:
         model = random_layered_model((n_int,) * 3, rng)
@@ -499,7 +500,7 @@ def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3,
         rc = thin_rcoef if name in ("thin", "learned") else None
         sim = _sim(*mdl, d, dt, tuple(c + lo for c in src), m, amps, npml, f, rc)
         runs[name] = (sim, lo)
-    learned = LearnedRun(runs["learned"][0], net)
+    learned = LearnedRun(runs["learned"][0], net) if runner is None else runner(runs["learned"][0])
     energy = {k: 0.0 for k in ("wide", "thin", "learned")}
     ref_norm = 0.0
     comps = ("vx", "vy", "vz")
@@ -535,3 +536,138 @@ def reflection_study(net: BandNet, n_int=40, steps=200, seed=0, d=10.0, dt=1e-3,
             torch.cuda.synchronize()
             out[f"ms_per_step_{name}"] = a.elapsed_time(b) / timing_steps
     return out
+
+
+# ------------------------------------------------------------ history study
+def _face_cells(b: SwBand, dev):
+    """Device-layout (z, y, x) index tensors [ncol, DEPTH] of the band cells of
+    every face, faces split as sw_band_gather splits them (x faces over the
+    whole interior face, y faces without the x bands, z faces without both)."""
+    lo, hi = (b.ox, b.oy, b.oz), (b.ox + b.ex, b.oy + b.ey, b.oz + b.ez)
+    full = [torch.arange(lo[a], hi[a], device=dev) for a in range(3)]
+    inner = [torch.arange(lo[a] + DEPTH, hi[a] - DEPTH, device=dev) for a in range(3)]
+    depth = lambda a, side: (torch.arange(lo[a], lo[a] + DEPTH, device=dev) if side == 0  # noqa: E731
+                             else torch.arange(hi[a] - DEPTH, hi[a], device=dev))
+    cells = []
+    for a in (0, 1, 2):  # x, y, z
+        other = [q for q in (0, 1, 2) if q != a]
+        rng_ = [full[q] if (a == 0 or (a == 1 and q == 2)) else inner[q] for q in other]
+        A, B = torch.meshgrid(rng_[0], rng_[1], indexing="ij")
+        A, B = A.reshape(-1, 1), B.reshape(-1, 1)
+        for side in (0, 1):
+            dd = depth(a, side)[None, :]
+            xyz = [None, None, None]
+            xyz[a] = dd.expand(A.shape[0], -1)
+            xyz[other[0]], xyz[other[1]] = A.expand(-1, DEPTH), B.expand(-1, DEPTH)
+            cells.append((xyz[2], xyz[1], xyz[0]))  # device layout [z][y][x]
+    return cells
+
+
+_FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def _band_values(engine, idx):
+    return torch.cat([engine.f[nm][idx] for nm in _FIELDS], 1)
+
+
+class LinearHistoryRun:
+    """Thin-CPML run whose band, after every step, gets X @ W added, X = [the
+    band after the step, the corrected bands before the last ``hist``
+    steps] (36 * (hist + 1) values per column), W one matrix per face."""
+
+    def __init__(self, sim, maps, hist):
+        self.sim, self.hist = sim, hist
+        self.cells = _face_cells(band_of(sim._e), sim._e.device)
+        self.maps = [m.float() for m in maps]
+        self.past = [[] for _ in self.cells]
+
+    def run(self, nsteps, it0=0):
+        e = self.sim._e
+        for n in range(it0, it0 + nsteps):
+            pre = [_band_values(e, c) for c in self.cells]
+            self.sim.run(1, n)
+            for f_, c in enumerate(self.cells):
+                self.past[f_] = (self.past[f_] + [pre[f_]])[-self.hist:]
+                if len(self.past[f_]) < self.hist:
+                    continue
+                X = torch.cat([_band_values(e, c)] + self.past[f_][::-1], 1)
+                corr = X @ self.maps[f_]
+                for q, nm in enumerate(_FIELDS):
+                    e.f[nm][c] += corr[:, q * DEPTH:(q + 1) * DEPTH]
+
+
+def history_study(n_runs=12, hist=4, n_val=2, ridges=(1e-8, 1e-5, 1e-3), alphas=(1.0, 0.5), seed=11,
+                  n_int=28, steps=150, test_seed=101, d=10.0, dt=1e-3, keep=0.2):
+    """What the one-step features lack, measured: with the corrected bands of
+    the last ``hist`` steps added to the band after the step, the one-step
+    band error of teacher-forced runs becomes almost a linear function of the
+    features (least squares per face, f64, held out on the last ``n_val``
+    runs), but the least-squares correction is not stable in closed loop.
+    Returns {"held_out": {face: removed fraction}, "closed_loop": {"ridge=r
+    alpha=a": reflected energy with the correction / thin layer's alone} on
+    the unseen model ``test_seed``} (removed = 1 - |T - XW|^2 / |T|^2; ridge
+    relative to the mean diagonal of the column-scaled normal matrix)."""
+    rng = np.random.default_rng(seed)
+    pools = [[] for _ in range(6)]
+    for _ in range(n_runs):
+        f = float(rng.uniform(10.0, 20.0))
+        nw, nt_ = grid_size(n_int, 20), grid_size(n_int, 4)
+        off = 16
+        vs, vp, rho = random_layered_model((nw, nw, nw), rng)
+        src = tuple(int(v) for v in rng.integers(22 + 6, 22 + n_int - 6, 3))
+        m = rng.standard_normal(6)
+        m /= np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2 + 2 * (m[3] ** 2 + m[4] ** 2 + m[5] ** 2))
+        amps = (1e6 * ricker2(np.arange(steps), dt, 1.0 / f, f, 1.0)).astype(np.float32)
+        W = _sim(vs, vp, rho, d, dt, src, m, amps, 20, f)
+        sl = slice(off, off + nt_)
+        T = _sim(vs[sl, sl, sl].copy(), vp[sl, sl, sl].copy(), rho[sl, sl, sl].copy(), d, dt,
+                 tuple(c - off for c in src), m, amps, 4, f)
+        bt, bw = band_of(T._e), band_of(W._e)
+        ct, cw = _face_cells(bt, T._e.device), _face_cells(bw, W._e.device)
+        ti = (slice(bt.oz, bt.oz + bt.ez), slice(bt.oy, bt.oy + bt.ey), slice(bt.ox, bt.ox + bt.ex))
+        wi = (slice(bw.oz, bw.oz + bw.ez), slice(bw.oy, bw.oy + bw.ey), slice(bw.ox, bw.ox + bw.ex))
+        gen = torch.Generator(device=T._e.device).manual_seed(int(rng.integers(1 << 31)))
+        past = [[] for _ in range(6)]
+        samples = [[] for _ in range(6)]
+        for n in range(steps):
+            for nm in _FIELDS:
+                T._e.f[nm][ti].copy_(W._e.f[nm][wi])
+            pre = [_band_values(T._e, c) for c in ct]
+            T.run(1, n)
+            W.run(1, n)
+            for f_ in range(6):
+                past[f_] = (past[f_] + [pre[f_]])[-hist:]
+                if n < 10 or len(past[f_]) < hist:
+                    continue
+                cur = _band_values(T._e, ct[f_])
+                X = torch.cat([cur] + past[f_][::-1], 1)
+                k = max(1, int(keep * X.shape[0]))
+                idx = torch.randint(0, X.shape[0], (k,), device=X.device, generator=gen)
+                samples[f_].append((X[idx].double(), (_band_values(W._e, cw[f_]) - cur)[idx].double()))
+        for f_ in range(6):
+            pools[f_].append((torch.cat([a for a, _ in samples[f_]]), torch.cat([b for _, b in samples[f_]])))
+
+    def fit(X, T, ridge):
+        sc = X.abs().amax(0).clamp_min(1e-300)
+        Xs = X / sc
+        A = Xs.T @ Xs
+        A = A + ridge * torch.trace(A) / A.shape[0] * torch.eye(A.shape[0], dtype=A.dtype, device=A.device)
+        return torch.linalg.solve(A, Xs.T @ T) / sc[:, None]
+
+    held = {}
+    for f_ in range(6):
+        Xt = torch.cat([p[0] for p in pools[f_][:-n_val]])
+        Tt = torch.cat([p[1] for p in pools[f_][:-n_val]])
+        Xv = torch.cat([p[0] for p in pools[f_][-n_val:]])
+        Tv = torch.cat([p[1] for p in pools[f_][-n_val:]])
+        Wt = fit(Xt, Tt, min(ridges))
+        held[FACES[f_]] = float(1 - torch.sum((Tv - Xv @ Wt) ** 2) / torch.sum(Tv ** 2))
+    loop = {}
+    for ridge in ridges:
+        maps = [fit(torch.cat([p[0] for p in pools[f_]]), torch.cat([p[1] for p in pools[f_]]), ridge)
+                for f_ in range(6)]
+        for alpha in alphas:
+            r = reflection_study(None, n_int=n_int, steps=steps, seed=test_seed,
+                                 runner=lambda sim: LinearHistoryRun(sim, [alpha * w for w in maps], hist))
+            loop[f"ridge={ridge:g} alpha={alpha:g}"] = r["learned"] / r["thin"]
+    return {"held_out": held, "closed_loop": loop}
