@@ -908,12 +908,27 @@ int sw_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, fl
     return launch_band_correct(g, b, params, prev, as_stream(stream));
 }
 
+// A cell in the band of the learned boundary (the kD outermost cells of the
+// interior box on each face), whose values the correction reads and writes.
+static bool in_band(const sw_band *b, int i, int j, int k)
+{
+    const int a = i - b->ox, c = j - b->oy, e = k - b->oz;
+    if (a < 0 || c < 0 || e < 0 || a >= b->ex || c >= b->ey || e >= b->ez) return false;
+    return a < SW_BAND_DEPTH || c < SW_BAND_DEPTH || e < SW_BAND_DEPTH || b->ex - 1 - a < SW_BAND_DEPTH ||
+           b->ey - 1 - c < SW_BAND_DEPTH || b->ez - 1 - e < SW_BAND_DEPTH;
+}
+
+// Steps with the band correction after every stress sweep.  A force source
+// is folded into the velocity sweep as in sw_ssg3d_run_ex; a stress source
+// into the previous step's stress sweep unless its cell is a band cell (the
+// correction after that sweep must see the cell without the next step's
+// injection).  Receivers are recorded after the correction (a receiver in the
+// band records the corrected value).
 int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, const sw_band *b,
                          const float *params, float *prev, int32_t it0, int32_t nsteps, void *stream)
 {
     int rc = check_ssg3d(g);
     if (rc) return rc;
-    const bool spml = g->cls == SW_CLASS_SPML;
     if ((rc = check_band(g, b))) return rc;
     SW_REQUIRE(params && prev, "null network or band state");
     SW_REQUIRE(nsteps >= 0 && it0 >= 0, "bad step range");
@@ -921,13 +936,20 @@ int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_recei
     if ((rc = check_source3d(g, src, it0)) || (rc = check_source3d(g, src, it0 + nsteps - 1))) return rc;
     if (rec && ((rc = check_receivers(rec, it0)) || (rc = check_receivers(rec, it0 + nsteps - 1)))) return rc;
     hipStream_t st = as_stream(stream);
-    const bool pre = src->kind == SW_SRC_MOMENT || src->kind == SW_SRC_EXPLOSIVE;
     const FieldPtrs fp = fields3d(g);
-    for (int it = it0; it < it0 + nsteps; ++it) {
-        if (pre) hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
-        if ((rc = spml ? launch_ssg3d_velocity_spml(g, st) : launch_ssg3d_velocity(g, st))) return rc;
-        if (!pre && src->kind != SW_SRC_NONE) hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
-        if ((rc = spml ? launch_ssg3d_stress_spml(g, st) : launch_ssg3d_stress(g, st))) return rc;
+    const int it_end = it0 + nsteps;
+    Fold3D plan = plan_fold3d(g, src, nullptr, 0);
+    if (plan.stress && in_band(b, src->ix, src->iy, src->iz)) plan.stress = 0;
+    Fold3D fo = plan;
+    for (int it = it0; it < it_end; ++it) {
+        fo.it_force = it;
+        fo.stress = it + 1 < it_end ? plan.stress : 0;
+        fo.it_next = it + 1;
+        if (stress_source(src) && (it == it0 || !plan.stress))
+            hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg3d_velocity_fold(g, fo, st))) return rc;
+        if (force_source(src) && !plan.force) hipLaunchKernelGGL(k_inject3d, dim3(1), dim3(1), 0, st, *g, *src, it);
+        if ((rc = launch_ssg3d_stress_fold(g, fo, st))) return rc;
         if ((rc = launch_band_correct(g, b, params, prev, st))) return rc;
         if (rec && (rc = record_launch(fp, rec, it, st))) return rc;
     }

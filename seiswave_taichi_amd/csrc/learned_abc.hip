@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
                 for (int r = 0; r < 4; ++r)
                     hs[(16 * wv + 4 * lk + r) * kHS + 16 * nb + li] = fast_tanh(acc[nb][r] + B1[16 * nb + li]);
         }
-        __syncthreads();
+        // layers 2 and 3 read only this wave's rows: no block barrier
+        __builtin_amdgcn_wave_barrier();
         // layer 2: h2 = tanh(W2^T h1 + b2), staged in xs
         {
             mf4 acc[4];
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
                 for (int r = 0; r < 4; ++r)
                     xs[(16 * wv + 4 * lk + r) * kXS + 16 * nb + li] = fast_tanh(acc[nb][r] + B2[16 * nb + li]);
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
         // layer 3: y = (W3^T h2 + b3) * out_scale * s, 36 outputs in 3 blocks
         // of 16 (the last 12 columns of B are zero), staged in hs
         {
