@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+MFMA_F32_PEAK_TFLOPS = 157.3  # MI355X dense f32 matrix-core peak
 
 
 def pml_params(vp_max, vs_max, npml, d, f, kmax=None):
@@ -413,6 +414,91 @@ def learned_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def learned_step_workload(args):
+    """Cost of the learned boundary's step on the 200^3 case: the thin 4-cell
+    CPML grid of its 155^3 interior (168^3) stepped with the band correction
+    after every stress sweep, for ``steps`` steps after ``warmup``, against
+    the same grid without the correction and the 20-cell CPML grid (200^3).
+    The network is the seeded initialisation with its output scale set to
+    zero: the kernel evaluates every layer of every column as always (its
+    cost does not depend on the weights), and the fields stay those of the
+    thin run, finite for any number of steps (an untrained network with a
+    non-zero output scale drives them to NaN within ~150 steps, and a NaN
+    state is no cost measurement).  The correction kernel is also timed alone: 136,838 band columns
+    per launch, each a 72-64-64-36 network (11,072 multiply-adds, the last
+    layer padded to 48 outputs on the matrix cores), so its roofline is the
+    f32 matrix-core rate."""
+    import ctypes as C
+
+    import torch
+
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd import learned_abc as LA
+    from seiswave_taichi_amd.coeffs import ricker2
+
+    torch.cuda.set_device(0)
+    n_int, d, dt, f = 155, 10.0, 1e-3, 10.0
+    nt = args.warmup + args.steps
+    amps = (1e6 * ricker2(np.arange(nt), dt, 1.2 / f, f, 1.0)).astype(np.float32)
+    mt = unit_mt(0)
+    sims = {}
+    for name, npml in (("wide_20_cell", 20), ("thin_4_cell", 4)):
+        n = LA.grid_size(n_int, npml)
+        mdl = [np.full((n,) * 3, v, np.float32) for v in (2000.0, 3500.0, 2200.0)]
+        c = n // 2
+        sims[name] = LA._sim(*mdl, d, dt, (c, c, c), mt, amps, npml, f)
+    net = LA.BandNet(seed=0)
+    a, b = net.lay["out_scale"]
+    net.params[:, a:b] = 0.0
+    run = LA.LearnedRun(sims["thin_4_cell"], net)
+    ev = lambda: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))  # noqa: E731
+    ms = {}
+    for name in ("wide_20_cell", "thin_4_cell", "thin_4_cell_plus_network"):
+        sim = sims["thin_4_cell" if name != "wide_20_cell" else name]
+        sim.reset()
+        run.reset()
+        go = (lambda k, i0: run.run(k, i0)) if name.endswith("network") else (lambda k, i0: sim.run(k, i0))
+        go(args.warmup, 0)
+        t0, t1 = ev()
+        t0.record()
+        go(args.steps, args.warmup)
+        t1.record()
+        torch.cuda.synchronize()
+        ms[name] = t0.elapsed_time(t1) / args.steps
+    e = sims["thin_4_cell"]._e
+    if not all(bool(torch.isfinite(t).all()) for t in e.f.values()):
+        raise RuntimeError("non-finite fields after the learned run")
+    lib, g, stream = L.load(), e.descriptor(), e.stream()
+    ncol = lib.sw_band_columns(C.byref(run.band))
+    for _ in range(5):
+        L.check(lib.sw_band_correct(g, C.byref(run.band), L.fptr(net.params), L.fptr(run.prev), stream), "correct")
+    evs = []
+    for _ in range(args.kernel_reps):  # each launch between its own events; the median counts
+        t0, t1 = ev()
+        t0.record()
+        L.check(lib.sw_band_correct(g, C.byref(run.band), L.fptr(net.params), L.fptr(run.prev), stream), "correct")
+        t1.record()
+        evs.append((t0, t1))
+    torch.cuda.synchronize()
+    k_ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
+    flops = 2.0 * ncol * (72 * 64 + 64 * 64 + 64 * 36)
+    flops_mfma = 2.0 * ncol * (72 * 64 + 64 * 64 + 64 * 48)
+    ach = flops / (k_ms / 1e3) / 1e12
+    out = {"metric": "learned boundary step cost", "value": round(ms["wide_20_cell"] / ms["thin_4_cell_plus_network"], 4),
+           "unit": "x (speedup of the learned step over the 20-cell CPML step)", "higher_is_better": True,
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "learned_200_step: 155^3 homogeneous interior of the 200^3 case; 20-cell CPML "
+                                  "(200^3) vs 4-cell CPML (168^3) with and without the band correction; "
+                                  "seeded untrained network"},
+           "ms_per_step": {k: round(v, 5) for k, v in ms.items()},
+           "band_columns": ncol,
+           "roofline": {"bound": "mfma", "kernel": "k_band_correct", "achieved": round(ach, 2),
+                        "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / MFMA_F32_PEAK_TFLOPS, 4),
+                        "flops_per_launch": flops, "mfma_flops_per_launch": flops_mfma,
+                        "launch_ms": round(k_ms, 5), "traffic": None}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,7 +506,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="tensor_200",
                     choices=["tensor_200", "spml_200", "layered_600x600x400", "shard_600x600x54", "gradient_800x300",
-                             "vs2d_400", "learned_200"])
+                             "vs2d_400", "learned_200", "learned_200_step"])
     ap.add_argument("--fd-cells", type=int, default=4,
                     help="gradient_800x300: cells per parameter of the finite-difference spot check")
     ap.add_argument("--kernel-reps", type=int, default=200)
@@ -440,6 +526,8 @@ def main():
         return vs2d_workload(args)
     if args.config == "learned_200":
         return learned_workload(args)
+    if args.config == "learned_200_step":
+        return learned_step_workload(args)
 
     import torch
 
