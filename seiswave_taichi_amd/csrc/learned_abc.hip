@@ -106,219 +106,253 @@ __global__ void k_band_gather(Fields9 F, int nx, int ny, BandGeo q, float *out)
 
 // Fused inference: gather, MLP, correct the fields, keep the corrected band.
 //
-// A block of 256 threads works on tiles of 64 columns of one face at a time:
-// the features go to LDS, and each layer is a 64 x n_out product on the
-// matrix cores (f32 16x16x4 MFMA, full f32 arithmetic), each wave owning 16
-// columns; the weights of the face sit in LDS and activations are re-staged
-// in LDS between layers.  Blocks take contiguous ranges of face-major tiles
-// and reload weights only when the face changes.
-constexpr int kCorrectThreads = 256;
-constexpr int kTile = 64;
+// Each wave works alone on tiles of 16 columns of one face, and a tile never
+// leaves the wave's registers.  Lane (li, lg) = lane li + 16 lg holds band
+// depth lg of column li of the tile.  The layers run on the matrix cores as
+// f32 16x16x4 products D[m][n] = sum_k A[m][k] B[k][n], with the weights as A
+// (m = output neuron within a block of 16) and the activations as B (n = the
+// tile's column): lane (li, lg) supplies A[li][lg] and B[lg][li] and receives
+// D[4 lg + r][li], r = 0..3.
+//  * Layer 1 takes its 72 inputs in the order (m, depth): k-step m (0..17)
+//    feeds input m of each depth (field m at the depth for m < 9, the change
+//    of field m - 9 for m >= 9), which is exactly what lane (li, lg) gathered
+//    for its own depth lg; W1's rows are permuted to match when the face's
+//    weights are staged in LDS.
+//  * A layer's output leaves lane (li, lg) with neurons 16 nb + 4 lg + r of
+//    column li; a 4x4 transpose over the lane groups lg and r (two
+//    permlane32 and two permlane16 swaps per block of 4 registers) gives it
+//    neurons 16 nb + 4 r + lg, the B operands of the next layer's k-steps
+//    16 nb + 4 r.
+//  * Layer 3's 36 outputs are laid out so that lane (li, lg) receives the 9
+//    corrections of its own depth: slot 16 nb + 4 g + r is output
+//    9 g + 4 nb + r (zero weights where 4 nb + r >= 9).
+// LDS holds only the face's permuted weights (52 KB), so no barrier follows
+// the one after staging them; while one wave issues matrix products the
+// other waves of its SIMD gather, evaluate tanh or scatter.  Each block serves
+// one face: the host gives every face a share of the blocks in proportion to
+// its tiles.
+constexpr int kWaveCols = 16;
+constexpr int kCorrectWaves = 8;
+constexpr int kCorrectThreads = 64 * kCorrectWaves;
 typedef float mf4 __attribute__((ext_vector_type(4)));
-constexpr int kW1End = kIn * kH;                  // end of W1 in a face's parameters
-constexpr int kW2Begin = kIn * kH + kH;           // W2 follows b1
-constexpr int kW2End = kW2Begin + kH * kH;
-static_assert(kParams % 4 == 0 && kW1End % 4 == 0 && kW2Begin % 4 == 0, "16-byte weight loads");
 
-// tanh from the hardware exponential: 1 - 2 / (e^{2x} + 1); absolute error
-// ~1e-7 (relative error grows only where tanh itself is tiny).
+// tanh from the hardware exponential and reciprocal: 1 - 2 / (e^{2x} + 1);
+// absolute error ~1e-7 (relative error grows only where tanh itself is tiny).
+// (A division here compiles to the full IEEE sequence, ten instructions.)
 __device__ __forceinline__ float fast_tanh(float x)
 {
-    return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f);
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f);
 }
-constexpr int kXS = kIn + 1;  // padded LDS row strides (odd: conflict-free column access)
-constexpr int kHS = kH + 1;
 
-struct TileMap {
-    int tile0[7];  // first tile of each face (face-major order)
+// Staged weights of one face (floats).  W1P[m][g][li][nb] = W1[in1(m, g)][16 nb + li];
+// W2P[k][li][nb] = W2[k][16 nb + li]; W3P[k][li][nb] = W3[k][out3(nb, li)] (zero
+// for nb = 3 and for the padding slots); B3P, OSP[16 nb + 4 g + r] = b3,
+// out_scale of output 9 g + 4 nb + r (zero for padding); ISI = 1 / in_scale.
+constexpr int kSW1 = 0, kSB1 = kSW1 + kIn * kH, kSW2 = kSB1 + kH, kSB2 = kSW2 + kH * kH, kSW3 = kSB2 + kH;
+constexpr int kSB3 = kSW3 + kH * 64, kSOS = kSB3 + 48, kSIS = kSOS + 48, kStaged = kSIS + kIn;
+// parameter offsets (params layout above)
+constexpr int kPB1 = kIn * kH, kPW2 = kPB1 + kH, kPB2 = kPW2 + kH * kH, kPW3 = kPB2 + kH, kPB3 = kPW3 + kH * kOut;
+constexpr int kPIS = kPB3 + kOut, kPOS = kPIS + kIn;
+static_assert(kPOS + kOut == kParams, "parameter layout");
+
+__device__ __forceinline__ int in1_of(int m, int g) { return m < kF ? kF * g + m : kOut + kF * g + (m - kF); }
+
+// Blocks of each face: blocks [blk0[f], blk0[f+1]) take the ntile[f] tiles of
+// face f in contiguous, near-equal ranges; the waves of a block take its
+// tiles in turn.
+struct BlockMap {
+    int blk0[7];
+    int ntile[6];
 };
 
-// One thread's share of a tile's band: depth gd of column gc (see below).
+// One lane's band cell of a tile: column li of the tile, depth lg.
 struct BandIO {
-    int f, gc, gd, col;
+    int col;
     bool live;
     long pc;
     float cur[kF], pv[kF];
 };
 
-// Global accesses (gather and scatter) use thread (gc, gd): on x faces the 4
-// band cells of a column are consecutive in x, so 4 neighbouring lanes take
-// the 4 depths of one column (16-byte runs); on y and z faces consecutive
-// lanes take consecutive columns, which are consecutive in x.
-__device__ __forceinline__ void fetch_band(BandIO &b, const Fields9 &F, int nx, int ny, const BandGeo &q,
-                                           const TileMap &tm, const float *prev, int tile)
+__device__ __forceinline__ void fetch_band(BandIO &b, const Fields9 &F, int nx, int ny, const BandGeo &q, int f,
+                                           const float *prev, int tile, int li, int lg)
 {
-    int f = 0;
-#pragma unroll
-    for (int u = 1; u < 6; ++u) f += (tile >= tm.tile0[u]);
-    const bool xface = f < 2;
-    b.f = f;
-    b.gc = xface ? (int)threadIdx.x >> 2 : (int)threadIdx.x % kTile;
-    b.gd = xface ? (int)threadIdx.x & 3 : (int)threadIdx.x / kTile;
-    const int r = (tile - tm.tile0[f]) * kTile + b.gc;  // column within the face
+    const int r = tile * kWaveCols + li;  // column within the face
     b.live = r < q.ncol[f];
     b.col = q.col0[f] + r;
     b.pc = 0;
     if (b.live) {
         const int na = lateral_a_extent(q, f);
         int i, j, k;
-        band_cell(q, f, r % na, r / na, b.gd, i, j, k);
+        band_cell(q, f, r % na, r / na, lg, i, j, k);
         b.pc = ((long)k * ny + j) * nx + i;
     }
 #pragma unroll
     for (int u = 0; u < kF; ++u) {
         b.cur[u] = b.live ? F.f[u][b.pc] : 0.f;
-        b.pv[u] = b.live ? prev[(long)b.col * kOut + b.gd * kF + u] : 0.f;
+        b.pv[u] = b.live ? prev[(long)b.col * kOut + lg * kF + u] : 0.f;
     }
 }
 
-__global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, TileMap tm,
+// Rows of 4 registers x 4 lane groups transposed (see above).
+__device__ __forceinline__ void transpose4(float *h)
+{
+    const auto s02 = __builtin_amdgcn_permlane32_swap(__float_as_uint(h[0]), __float_as_uint(h[2]), false, false);
+    const auto s13 = __builtin_amdgcn_permlane32_swap(__float_as_uint(h[1]), __float_as_uint(h[3]), false, false);
+    const auto t01 = __builtin_amdgcn_permlane16_swap(s02[0], s13[0], false, false);
+    const auto t23 = __builtin_amdgcn_permlane16_swap(s02[1], s13[1], false, false);
+    h[0] = __uint_as_float(t01[0]);
+    h[1] = __uint_as_float(t01[1]);
+    h[2] = __uint_as_float(t23[0]);
+    h[3] = __uint_as_float(t23[1]);
+}
+
+__global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, BlockMap bm,
                                                                   const float *params, float *prev)
 {
-    __shared__ __attribute__((aligned(16))) float w[kParams];
-    __shared__ float xs[kTile * kXS];  // features, then h2
-    __shared__ float hs[kTile * kHS];  // h1, then the correction
-    __shared__ float red[4][kTile];
-    __shared__ float scale[kTile];
-    const int ntile = tm.tile0[6];
-    const int t0 = (int)(((long)blockIdx.x * ntile) / gridDim.x);
-    const int t1 = (int)(((long)(blockIdx.x + 1) * ntile) / gridDim.x);
-    const int c = threadIdx.x % kTile, grp = threadIdx.x / kTile;
-    int loaded = -1;
+    __shared__ __attribute__((aligned(16))) float w[kStaged];
+    const int b = (int)blockIdx.x;
+    int f = 0;
+#pragma unroll
+    for (int u = 1; u < 6; ++u) f += (b >= bm.blk0[u]);
+    const int nb_f = bm.blk0[f + 1] - bm.blk0[f], lb = b - bm.blk0[f];
+    const int t0 = (int)(((long)lb * bm.ntile[f]) / nb_f);
+    const int t1 = (int)(((long)(lb + 1) * bm.ntile[f]) / nb_f);
+    {
+        // All of a thread's loads are issued before its first LDS store, so
+        // the ~26 L2 reads per thread overlap instead of running one by one.
+        const float *P = params + (long)f * kParams;
+        constexpr int kPer = (kStaged + kCorrectThreads - 1) / kCorrectThreads;
+        float x[kPer];
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int v = (int)threadIdx.x + t * kCorrectThreads;
+            int src = -1;  // parameter index, or -1 for a zero (padding)
+            bool inv = false;
+            if (v < kSB1) {  // W1P
+                const int nb = v & 3, li = (v >> 2) & 15, g = (v >> 6) & 3, m = v >> 8;
+                src = in1_of(m, g) * kH + 16 * nb + li;
+            } else if (v < kSW2) {
+                src = kPB1 + (v - kSB1);
+            } else if (v < kSB2) {  // W2P
+                const int e = v - kSW2, nb = e & 3, li = (e >> 2) & 15, k = e >> 6;
+                src = kPW2 + k * kH + 16 * nb + li;
+            } else if (v < kSW3) {
+                src = kPB2 + (v - kSB2);
+            } else if (v < kSB3) {  // W3P
+                const int e = v - kSW3, nb = e & 3, li = (e >> 2) & 15, k = e >> 6;
+                const int u = 4 * nb + (li & 3);
+                if (nb < 3 && u < kF) src = kPW3 + k * kOut + kF * (li >> 2) + u;
+            } else if (v < kSIS) {  // B3P, OSP
+                const int e = (v - kSB3) % 48, nb = e >> 4, g = (e >> 2) & 3, r = e & 3, u = 4 * nb + r;
+                if (u < kF) src = (v < kSOS ? kPB3 : kPOS) + kF * g + u;
+            } else if (v < kStaged) {  // 1 / in_scale: the inputs are scaled with one multiply each
+                src = kPIS + (v - kSIS);
+                inv = true;
+            }
+            const float y = src >= 0 ? P[src] : 0.f;
+            x[t] = inv ? 1.f / y : y;
+        }
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int v = (int)threadIdx.x + t * kCorrectThreads;
+            if (v < kStaged) w[v] = x[t];
+        }
+    }
+    __syncthreads();
+    const float *W1P = w + kSW1, *B1 = w + kSB1, *W2P = w + kSW2, *B2 = w + kSB2, *W3P = w + kSW3;
+    const float *B3P = w + kSB3, *OSP = w + kSOS, *ISI = w + kSIS;
+    const int wv = (int)threadIdx.x / 64, ln = (int)threadIdx.x % 64;
+    const int li = ln % 16, lg = ln / 16;
     BandIO io, nx_io;
-    if (t0 < t1) fetch_band(io, F, nx, ny, q, tm, prev, t0);
-    for (int tile = t0; tile < t1; ++tile) {
-        const int f = io.f;
-        __syncthreads();  // the previous tile is done with w, xs and hs
-        if (f != loaded) {
-            const float *P = params + (long)f * kParams;
-            // W1 and W2 (64 outputs each) are stored permuted, [row][li][nb] =
-            // W[row][16 nb + li], so one 16-byte LDS read gives a lane its
-            // B operands of all 4 output blocks; the rest as in params.
-            // 16-byte global loads, 4 in flight per thread.
-            const mf4 *P4 = reinterpret_cast<const mf4 *>(P);
-#pragma unroll 4
-            for (int v = threadIdx.x; v < kParams / 4; v += blockDim.x) {
-                const mf4 x = P4[v];
+    if (t0 + wv < t1) fetch_band(io, F, nx, ny, q, f, prev, t0 + wv, li, lg);
+    for (int tile = t0 + wv; tile < t1; tile += kCorrectWaves) {
+        // the staged weights are read where they are used: without this the
+        // compiler hoists all of them out of the tile loop into registers
+        asm volatile("" ::: "memory");
+        // 1. inputs of depth lg of column li and the column's scale s, from
+        // the four depths' partial sums added in depth order
+        float xin[2 * kF];
+        float ss = 0.f;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int u = 4 * v + e;
-                    int dst = u;
-                    if (u < kW1End || (u >= kW2Begin && u < kW2End)) {
-                        const int base = u < kW1End ? 0 : kW2Begin;
-                        const int row = (u - base) / kH, col = (u - base) % kH;
-                        dst = base + row * kH + 4 * (col % 16) + col / 16;
-                    }
-                    w[dst] = x[e];
-                }
-            }
-            loaded = f;
-            __syncthreads();
+        for (int u = 0; u < kF; ++u) {
+            xin[u] = io.cur[u] * ISI[kF * lg + u];
+            xin[kF + u] = (io.cur[u] - io.pv[u]) * ISI[kOut + kF * lg + u];
+            ss += xin[u] * xin[u] + xin[kF + u] * xin[kF + u];
         }
-        const float *W1 = w, *B1 = W1 + kIn * kH, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH;
-        const float *B3 = W3 + kH * kOut, *IS = B3 + kOut, *OS = IS + kIn;
-        // 1. features of band depth gd of column gc (fetched ahead)
-        {
-            float ss = 0.f;
-#pragma unroll
-            for (int u = 0; u < kF; ++u) {
-                const int o = io.gd * kF + u;
-                const float x0 = io.cur[u] / IS[o];
-                const float x1 = (io.cur[u] - io.pv[u]) / IS[kOut + o];
-                xs[io.gc * kXS + o] = x0;
-                xs[io.gc * kXS + kOut + o] = x1;
-                ss += x0 * x0 + x1 * x1;
-            }
-            red[io.gd][io.gc] = ss;
-        }
-        __syncthreads();
-        if (grp == 0) scale[c] = sqrtf((red[0][c] + red[1][c] + red[2][c] + red[3][c]) / (float)kIn);
-        __syncthreads();
-        // the next tile's band loads overlap this tile's layers
-        if (tile + 1 < t1) fetch_band(nx_io, F, nx, ny, q, tm, prev, tile + 1);
-        // 2-4. The three layers as 16x16x4 f32 matrix-core products: wave wv
-        // owns tile rows (columns) 16 wv .. 16 wv + 15; lane (li, lk) feeds
-        // A[li][lk] = act[16 wv + li][k0 + lk] and B[lk][li] = W[k0 + lk][n0 + li],
-        // and receives D[4 lk + r][li], r = 0..3.  Rows are private to a wave.
-        const int wv = (int)threadIdx.x / 64, ln = (int)threadIdx.x % 64;
-        const int li = ln % 16, lk = ln / 16;
-        const int arow = 16 * wv + li;  // A row of this lane
-        const float ainv = scale[arow] > 0.f ? 1.f / scale[arow] : 0.f;
-        // layer 1: h1 = tanh(W1^T (u / s) + b1), 64 outputs in 4 blocks of 16
+        const float sc = sqrtf((__shfl(ss, li) + __shfl(ss, li + 16) + __shfl(ss, li + 32) + __shfl(ss, li + 48)) /
+                               (float)kIn);
+        const float ainv = sc > 0.f ? 1.f / sc : 0.f;
+        // the wave's next tile: its band loads overlap this tile's layers
+        if (tile + kCorrectWaves < t1) fetch_band(nx_io, F, nx, ny, q, f, prev, tile + kCorrectWaves, li, lg);
+        // 2. layer 1: h1 = tanh(W1^T (u / s) + b1)
+        float h[4][4];
         {
             mf4 acc[4];
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) acc[nb] = mf4{0.f, 0.f, 0.f, 0.f};
-            for (int k0 = 0; k0 < kIn; k0 += 4) {
-                const float av = xs[arow * kXS + k0 + lk] * ainv;
-                const mf4 bv = *reinterpret_cast<const mf4 *>(W1 + (k0 + lk) * kH + 4 * li);
+#pragma unroll
+            for (int m = 0; m < 2 * kF; ++m) {
+                const float bv = xin[m] * ainv;
+                const mf4 av = *reinterpret_cast<const mf4 *>(W1P + ((m * 4 + lg) * 16 + li) * 4);
 #pragma unroll
                 for (int nb = 0; nb < 4; ++nb)
-                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[nb], 0, 0, 0);
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[nb], bv, acc[nb], 0, 0, 0);
             }
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
+            for (int nb = 0; nb < 4; ++nb) {
+                const mf4 bias = *reinterpret_cast<const mf4 *>(B1 + 16 * nb + 4 * lg);
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    hs[(16 * wv + 4 * lk + r) * kHS + 16 * nb + li] = fast_tanh(acc[nb][r] + B1[16 * nb + li]);
+                for (int r = 0; r < 4; ++r) h[nb][r] = fast_tanh(acc[nb][r] + bias[r]);
+                transpose4(h[nb]);
+            }
         }
-        // layers 2 and 3 read only this wave's rows: no block barrier
-        __builtin_amdgcn_wave_barrier();
-        // layer 2: h2 = tanh(W2^T h1 + b2), staged in xs
+        // 3. layer 2: h2 = tanh(W2^T h1 + b2)
         {
             mf4 acc[4];
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) acc[nb] = mf4{0.f, 0.f, 0.f, 0.f};
-            for (int k0 = 0; k0 < kH; k0 += 4) {
-                const float av = hs[arow * kHS + k0 + lk];
-                const mf4 bv = *reinterpret_cast<const mf4 *>(W2 + (k0 + lk) * kH + 4 * li);
 #pragma unroll
-                for (int nb = 0; nb < 4; ++nb)
-                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nb], acc[nb], 0, 0, 0);
-            }
+            for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const int k0 = 16 * kb + 4 * s4;
+                    const mf4 av = *reinterpret_cast<const mf4 *>(W2P + ((k0 + lg) * 16 + li) * 4);
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    xs[(16 * wv + 4 * lk + r) * kXS + 16 * nb + li] = fast_tanh(acc[nb][r] + B2[16 * nb + li]);
-        }
-        __builtin_amdgcn_wave_barrier();
-        // layer 3: y = (W3^T h2 + b3) * out_scale * s, 36 outputs in 3 blocks
-        // of 16 (the last 12 columns of B are zero), staged in hs
-        {
-            mf4 acc[3];
-#pragma unroll
-            for (int nb = 0; nb < 3; ++nb) acc[nb] = mf4{0.f, 0.f, 0.f, 0.f};
-            for (int k0 = 0; k0 < kH; k0 += 4) {
-                const float av = xs[arow * kXS + k0 + lk];
-#pragma unroll
-                for (int nb = 0; nb < 3; ++nb) {
-                    const int o = 16 * nb + li;
-                    const float bv = o < kOut ? W3[(k0 + lk) * kOut + o] : 0.f;
-                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[nb], 0, 0, 0);
+                    for (int nb = 0; nb < 4; ++nb)
+                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[nb], h[kb][s4], acc[nb], 0, 0, 0);
                 }
-            }
 #pragma unroll
-            for (int nb = 0; nb < 3; ++nb) {
-                const int o = 16 * nb + li;
-                if (o < kOut)
+            for (int nb = 0; nb < 4; ++nb) {
+                const mf4 bias = *reinterpret_cast<const mf4 *>(B2 + 16 * nb + 4 * lg);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * wv + 4 * lk + r;
-                        const float sr = scale[row];
-                        hs[row * kHS + o] = sr > 0.f ? (acc[nb][r] + B3[o]) * OS[o] * sr : 0.f;
-                    }
+                for (int r = 0; r < 4; ++r) h[nb][r] = fast_tanh(acc[nb][r] + bias[r]);
+                transpose4(h[nb]);
             }
         }
-        __syncthreads();
-        // 5. add the correction to band depth gd of column gc, keep the band
+        // 4. layer 3: y = (W3^T h2 + b3) * out_scale * s, the 9 outputs of
+        // depth lg in slots (nb, r) with 4 nb + r < 9
+        mf4 acc3[3];
+#pragma unroll
+        for (int nb = 0; nb < 3; ++nb) acc3[nb] = mf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int k0 = 16 * kb + 4 * s4;
+                const mf4 av = *reinterpret_cast<const mf4 *>(W3P + ((k0 + lg) * 16 + li) * 4);
+#pragma unroll
+                for (int nb = 0; nb < 3; ++nb)
+                    acc3[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[nb], h[kb][s4], acc3[nb], 0, 0, 0);
+            }
+        // 5. add the correction to depth lg of column li, keep the band
         if (io.live) {
 #pragma unroll
             for (int u = 0; u < kF; ++u) {
-                const int o = io.gd * kF + u;
-                const float v = io.cur[u] + hs[io.gc * kHS + o];
+                const int nb = u / 4, r = u % 4, slot = 16 * nb + 4 * lg + r;
+                const float y = sc > 0.f ? (acc3[nb][r] + B3P[slot]) * OSP[slot] * sc : 0.f;
+                const float v = io.cur[u] + y;
                 F.f[u][io.pc] = v;
-                prev[(long)io.col * kOut + o] = v;
+                prev[(long)io.col * kOut + lg * kF + u] = v;
             }
         }
         io = nx_io;
@@ -427,14 +461,23 @@ int launch_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, hipStrea
 int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, hipStream_t st)
 {
     const BandGeo q = band_geo(*b);
-    TileMap tm;
-    tm.tile0[0] = 0;
-    for (int f = 0; f < 6; ++f) tm.tile0[f + 1] = tm.tile0[f] + (q.ncol[f] + kTile - 1) / kTile;
-    if (tm.tile0[6] == 0) return SW_OK;
-    // two resident blocks per CU (80 KB of LDS each), a few tiles per block
-    const int nblock = std::min(tm.tile0[6], 512);
-    hipLaunchKernelGGL(k_band_correct, dim3(nblock), dim3(kCorrectThreads), 0, st, fields9(*g), g->nx, g->ny, q, tm,
-                       params, prev);
+    BlockMap bm;
+    long total = 0;
+    for (int f = 0; f < 6; ++f) {
+        bm.ntile[f] = (q.ncol[f] + kWaveCols - 1) / kWaveCols;
+        total += bm.ntile[f];
+    }
+    if (total == 0) return SW_OK;
+    // about two tiles per wave, at most two blocks per CU of MI355X's 256
+    const long want = std::max(1L, std::min(512L, (total + 2 * kCorrectWaves - 1) / (2 * kCorrectWaves)));
+    bm.blk0[0] = 0;
+    for (int f = 0; f < 6; ++f) {
+        long nbf = bm.ntile[f] ? (want * bm.ntile[f] + total - 1) / total : 0;
+        nbf = std::min<long>(std::max<long>(nbf, bm.ntile[f] ? 1 : 0), bm.ntile[f]);
+        bm.blk0[f + 1] = bm.blk0[f] + (int)nbf;
+    }
+    hipLaunchKernelGGL(k_band_correct, dim3(bm.blk0[6]), dim3(kCorrectThreads), 0, st, fields9(*g), g->nx, g->ny, q,
+                       bm, params, prev);
     return launch_status("band correction");
 }
 
