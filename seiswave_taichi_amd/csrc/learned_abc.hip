@@ -128,11 +128,14 @@ __global__ void k_band_gather(Fields9 F, int nx, int ny, BandGeo q, float *out)
 //    9 g + 4 nb + r (zero weights where 4 nb + r >= 9).
 // LDS holds only the face's permuted weights (52 KB), so no barrier follows
 // the one after staging them; while one wave issues matrix products the
-// other waves of its SIMD gather, evaluate tanh or scatter.  Each block serves
-// one face: the host gives every face a share of the blocks in proportion to
-// its tiles.
+// other waves of its SIMD gather, evaluate tanh or scatter.  Blocks of 10
+// waves at 5 waves per SIMD (96 VGPRs: the next tile is not prefetched) put
+// 5,120 waves on MI355X's 256 CUs, so the 8,553 tiles of the 200^3 case
+// take at most two per wave (at 4 waves per SIMD, 3 for some: measured 62.7
+// against 58.7 us).  Each block serves one face: the host gives every face a
+// share of the blocks in proportion to its tiles.
 constexpr int kWaveCols = 16;
-constexpr int kCorrectWaves = 8;
+constexpr int kCorrectWaves = 10;
 constexpr int kCorrectThreads = 64 * kCorrectWaves;
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
@@ -206,7 +209,7 @@ __device__ __forceinline__ void transpose4(float *h)
     h[3] = __uint_as_float(t23[1]);
 }
 
-__global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, BlockMap bm,
+__global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_per_eu(5))) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, BlockMap bm,
                                                                   const float *params, float *prev)
 {
     __shared__ __attribute__((aligned(16))) float w[kStaged];
@@ -263,9 +266,9 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
     const float *B3P = w + kSB3, *OSP = w + kSOS, *ISI = w + kSIS;
     const int wv = (int)threadIdx.x / 64, ln = (int)threadIdx.x % 64;
     const int li = ln % 16, lg = ln / 16;
-    BandIO io, nx_io;
-    if (t0 + wv < t1) fetch_band(io, F, nx, ny, q, f, prev, t0 + wv, li, lg);
+    BandIO io;
     for (int tile = t0 + wv; tile < t1; tile += kCorrectWaves) {
+        fetch_band(io, F, nx, ny, q, f, prev, tile, li, lg);
         // the staged weights are read where they are used: without this the
         // compiler hoists all of them out of the tile loop into registers
         asm volatile("" ::: "memory");
@@ -282,8 +285,6 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
         const float sc = sqrtf((__shfl(ss, li) + __shfl(ss, li + 16) + __shfl(ss, li + 32) + __shfl(ss, li + 48)) /
                                (float)kIn);
         const float ainv = sc > 0.f ? 1.f / sc : 0.f;
-        // the wave's next tile: its band loads overlap this tile's layers
-        if (tile + kCorrectWaves < t1) fetch_band(nx_io, F, nx, ny, q, f, prev, tile + kCorrectWaves, li, lg);
         // 2. layer 1: h1 = tanh(W1^T (u / s) + b1)
         float h[4][4];
         {
@@ -355,7 +356,6 @@ __global__ void __launch_bounds__(kCorrectThreads) k_band_correct(Fields9 F, int
                 prev[(long)io.col * kOut + lg * kF + u] = v;
             }
         }
-        io = nx_io;
     }
 }
 
@@ -469,6 +469,7 @@ int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params
     }
     if (total == 0) return SW_OK;
     // about two tiles per wave, at most two blocks per CU of MI355X's 256
+    // (one wave of blocks)
     const long want = std::max(1L, std::min(512L, (total + 2 * kCorrectWaves - 1) / (2 * kCorrectWaves)));
     bm.blk0[0] = 0;
     for (int f = 0; f < 6; ++f) {
