@@ -147,3 +147,54 @@ def test_fused_correction_matches_oracle():
     # a column whose features are all zero gets no correction
     assert np.all(np.isfinite(after.cpu().numpy()))
     np.testing.assert_array_equal(prev.cpu().numpy(), after.cpu().numpy())
+
+
+@pytest.mark.parametrize("where", ["interior", "band"])
+def test_learned_run_equals_per_step_composition(where):
+    """sw_ssg3d_run_learned (which folds a moment source into the previous
+    step's stress sweep when its cell is not a band cell) against the same
+    steps composed from the per-step entry points: inject, velocity sweep,
+    stress sweep, band correction, receiver record.  Bit for bit: fields,
+    band state and trace."""
+    from seiswave_taichi_amd import _lib as L
+    from seiswave_taichi_amd import learned_abc as LA
+    from seiswave_taichi_amd.coeffs import ricker2
+    lib = LA._sig()
+    n_int, thin, steps = 20, 4, 40
+    n = LA.grid_size(n_int, thin)
+    vs, vp, rho = LA.random_layered_model((n, n, n), np.random.default_rng(5))
+    lo = 2 + thin  # first interior cell, band depth 0
+    cell = (n // 2,) * 3 if where == "interior" else (lo + 1, n // 2 - 1, n // 2 + 2)
+    amps = (1e3 * ricker2(np.arange(steps), 1e-3, 1.2 / 15.0, 15.0, 1.0)).astype(np.float32)
+    w = LA._sim(vs, vp, rho, 10.0, 1e-3, cell, [0.6, -0.3, 0.2, 0.4, -0.1, 0.3], amps, thin, 15.0)
+    net = _net_params(7)
+    a, b = net.lay["out_scale"]
+    # A random network scaled so the run stays bounded: its features mix stresses and velocities
+    # (about 1e6 apart) with random unit scales, and from an output-scale factor of ~1e-3 up the
+    # closed loop diverges within 20 steps (the earlier and the current kernel alike); at 1e-6 the
+    # correction still moves the band state by a few per cent.
+    net.params[:, a:b] *= 1e-6
+    run = LA.LearnedRun(w, net)
+    e = w._e
+    band = run.band
+    inside = lambda c: min(min(c[d] - lo, (lo + n_int - 1) - c[d]) for d in range(3)) < 4  # noqa: E731
+    assert inside(cell) == (where == "band")
+    run.run(steps, 0)
+    torch.cuda.synchronize()
+    got = {k: v.clone() for k, v in e.f.items()}
+    got_prev, got_trace = run.prev.clone(), w._rec.data_t.clone()
+    w.reset()
+    run.reset()
+    g, st = e.descriptor(), e.stream()
+    for it in range(steps):
+        L.check(lib.sw_ssg3d_inject(g, w._src.s, it, st), "inject")
+        L.check(lib.sw_ssg3d_velocity(g, st), "velocity")
+        L.check(lib.sw_ssg3d_stress(g, st), "stress")
+        L.check(lib.sw_band_correct(g, C.byref(band), L.fptr(net.params), L.fptr(run.prev), st), "correct")
+        L.check(lib.sw_ssg3d_record(g, w._rec.r, it, st), "record")
+    torch.cuda.synchronize()
+    assert float(got_trace.abs().max()) > 0 and bool(torch.isfinite(got_trace).all())
+    for k in got:
+        assert torch.equal(got[k], e.f[k]), k
+    assert torch.equal(got_prev, run.prev)
+    assert torch.equal(got_trace, w._rec.data_t)
