@@ -325,7 +325,11 @@ int sw_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, void *stream
 /* fused inference: fields += MLP correction on every band column; prev[c][36]
  * holds the corrected band of the previous step and is updated in place */
 int sw_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, void *stream);
-/* nsteps of sw_ssg3d_run with sw_band_correct after each stress sweep */
+/* nsteps of sw_ssg3d_run with sw_band_correct after each stress sweep; the
+ * receivers are recorded after the correction (a receiver on a band cell
+ * records the corrected value).  Equal, bit for bit, to calling per step
+ * sw_ssg3d_inject, sw_ssg3d_velocity, sw_ssg3d_stress, sw_band_correct and
+ * sw_ssg3d_record. */
 int sw_ssg3d_run_learned(const sw_ssg3d *g, const sw_source *src, const sw_receivers *rec, const sw_band *b,
                          const float *params, float *prev, int32_t it0, int32_t nsteps, void *stream);
 /* training of one face network on a batch X[B][72] -> targets T[B][36] */
