@@ -212,6 +212,34 @@ def stream_ceiling_gbs(n, reps=20, mixes=STREAM_MIXES):
     return out
 
 
+def mfma_ceiling_tflops(steps=2048, reps=3):
+    """Rate of the library's matrix-core probe (sw_mfma_probe: chains of f32
+    16x16x4 MFMA products, four independent accumulators per wave, nothing
+    else) at several grid sizes, the practical ceiling of the learned
+    boundary's correction kernel.  Returns {"TFLOPs": best, "by_blocks": {...}}."""
+    import ctypes as C
+
+    import torch
+
+    from seiswave_taichi_amd import _lib as L
+    lib = L.load()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rates = {}
+    for nblock in (1024, 2048, 4096, 8192):
+        out = torch.empty(nblock * 256, dtype=torch.float32, device="cuda")
+        L.check(lib.sw_mfma_probe(L.fptr(out), nblock, steps, stream), "matrix-core probe")
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            L.check(lib.sw_mfma_probe(L.fptr(out), nblock, steps, stream), "matrix-core probe")
+        b.record()
+        torch.cuda.synchronize()
+        t = a.elapsed_time(b) / 1e3 / reps
+        flops = nblock * 4 * steps * 4 * 2048.0  # waves x rounds x products x flop
+        rates[nblock] = round(flops / t / 1e12, 1)
+        del out
+    return {"TFLOPs": max(rates.values()), "by_blocks": rates}
+
 def cpu_baseline(w, budget_s=15.0):
     """Oracle (C restatement, OpenMP) on a bounded sample of the same workload."""
     from oracle import oracle as O
@@ -496,6 +524,10 @@ def learned_step_workload(args):
                         "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / MFMA_F32_PEAK_TFLOPS, 4),
                         "flops_per_launch": flops, "mfma_flops_per_launch": flops_mfma,
                         "launch_ms": round(k_ms, 5), "traffic": None}}
+    probe = mfma_ceiling_tflops()
+    out["roofline"]["mfma_probe_TFLOPs"] = probe["TFLOPs"]
+    out["roofline"]["frac_of_mfma_probe"] = round(flops_mfma / (k_ms / 1e3) / 1e12 / probe["TFLOPs"], 4)
+    out["mfma_probe"] = probe
     print(json.dumps(out), flush=True)
 
 

@@ -363,6 +363,10 @@ int sw_mlp_loss_backward_noalloc(const float *params, const float *X, const floa
  * multiple of 4, arrays 16-byte aligned and device resident. */
 int sw_stream_probe(const float *const *in, int32_t nin, float *const *out, int32_t nout, int64_t n,
                     int32_t nblock, void *stream);
+/* Matrix-core probe: nblock blocks of 256 threads, each wave issuing `steps`
+ * rounds of 4 independent f32 16x16x4 MFMA products (2048 flop each); out
+ * receives one float per thread (nblock * 256 floats, device resident). */
+int sw_mfma_probe(float *out, int32_t nblock, int32_t steps, void *stream);
 
 #ifdef __cplusplus
 }

@@ -120,6 +120,7 @@ SIGNATURES = {
     "sw_mlp_loss_backward": [_fp] * 6 + [C.c_int32, _fp, _fp, _fp, C.c_void_p],
     "sw_mlp_loss_backward_noalloc": [_fp] * 6 + [C.c_int32, _fp, _fp, C.c_void_p],
     "sw_stream_probe": [C.POINTER(_fp), C.c_int32, C.POINTER(_fp), C.c_int32, C.c_int64, C.c_int32, C.c_void_p],
+    "sw_mfma_probe": [_fp, C.c_int32, C.c_int32, C.c_void_p],
     "sw_adam": [_fp, _fp, _fp, _fp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
                 C.c_void_p],
 }

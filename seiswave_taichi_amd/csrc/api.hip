@@ -41,6 +41,7 @@ int launch_adam(float *p, const float *g, float *m, float *v, int n, float lr, f
 int mlp_param_count();
 int launch_stream_probe(const float *const *in, int nin, float *const *out, int nout, long n, int nblock,
                         hipStream_t st);
+int launch_mfma_probe(float *out, int nblock, int steps, hipStream_t st);
 int mlp_scratch_lookup(float **out);
 int launch_mlp_loss_backward_noalloc(const float *params, const float *X, const float *H1, const float *H2,
                                      const float *Y, const float *T, int B, float *grad, float *loss_rows,
@@ -967,6 +968,12 @@ int sw_stream_probe(const float *const *in, int32_t nin, float *const *out, int3
         SW_REQUIRE(out[o] && reinterpret_cast<uintptr_t>(out[o]) % 16 == 0, "stream probe output %d not 16-byte aligned",
                    o);
     return launch_stream_probe(in, nin, out, nout, n, nblock, as_stream(stream));
+}
+
+int sw_mfma_probe(float *out, int32_t nblock, int32_t steps, void *stream)
+{
+    SW_REQUIRE(out != nullptr && nblock > 0 && steps > 0, "bad matrix-core probe arguments");
+    return launch_mfma_probe(out, nblock, steps, as_stream(stream));
 }
 
 int sw_mlp_forward(const float *params, const float *X, int32_t B, float *H1, float *H2, float *Y, void *stream)

@@ -1,7 +1,14 @@
+// Measurement probes (no counterpart in the reference).
+//
 // Streaming probe: a kernel that moves exactly the bytes of a sweep's access
 // mix (nin arrays read, nout arrays written, every element once) and does no
 // stencil work, so its rate is the practical HBM ceiling the sweeps are
-// measured against.  No counterpart in the reference.
+// measured against.
+//
+// Matrix-core probe: chains of f32 16x16x4 MFMA products, the instruction of
+// the learned boundary's correction kernel, with four independent
+// accumulators per wave and nothing else, so its rate is the practical
+// matrix-core ceiling that kernel is measured against.
 //
 // Each thread handles kVec consecutive floats per array (16-byte loads and
 // stores) and walks the arrays with a grid stride; the host picks the grid.
@@ -39,7 +46,34 @@ void launch(const StreamArgs &a, long n4, int nblock, hipStream_t st)
     hipLaunchKernelGGL((k_stream<NIN, NOUT>), dim3(nblock), dim3(256), 0, st, a, n4);
 }
 
+// Each wave issues `steps` rounds of 4 independent MFMA products; one float
+// per lane of the result is written so the products are kept.
+__global__ void __launch_bounds__(256) k_mfma_probe(float *out, int steps)
+{
+    typedef float mf4 __attribute__((ext_vector_type(4)));
+    mf4 acc[4] = {mf4{0.f, 0.f, 0.f, 0.f}, mf4{0.f, 0.f, 0.f, 0.f}, mf4{0.f, 0.f, 0.f, 0.f},
+                  mf4{0.f, 0.f, 0.f, 0.f}};
+    // lane-distinct operands (lane l supplies A = 1 + l/100, B = 1 - l/200), so
+    // the result also shows the operand layout
+    const int l = (int)threadIdx.x % 64;
+    const float a = 1.f + 0.01f * (float)l, b = 1.f - 0.005f * (float)l;
+    for (int i = 0; i < steps; ++i) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[k], 0, 0, 0);
+    }
+    float v = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    out[(long)blockIdx.x * blockDim.x + threadIdx.x] = v;
+}
+
 }  // namespace
+
+int launch_mfma_probe(float *out, int nblock, int steps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mfma_probe, dim3(nblock), dim3(256), 0, st, out, steps);
+    return launch_status("matrix-core probe");
+}
 
 int launch_stream_probe(const float *const *in, int nin, float *const *out, int nout, long n, int nblock,
                         hipStream_t st)
