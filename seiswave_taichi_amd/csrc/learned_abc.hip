@@ -458,6 +458,22 @@ int launch_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, hipStrea
     return launch_status("band gather");
 }
 
+// Compute units of the current device, looked up once per device (256 on
+// MI355X; 256 if the query fails).
+static int compute_units()
+{
+    static std::mutex mu;
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lock(mu);
+    if (cus[dev] == 0) {
+        int v = 0;
+        cus[dev] = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0 ? v : 256;
+    }
+    return cus[dev];
+}
+
 int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, hipStream_t st)
 {
     const BandGeo q = band_geo(*b);
@@ -468,9 +484,9 @@ int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params
         total += bm.ntile[f];
     }
     if (total == 0) return SW_OK;
-    // about two tiles per wave, at most two blocks per CU of MI355X's 256
-    // (one wave of blocks)
-    const long want = std::max(1L, std::min(512L, (total + 2 * kCorrectWaves - 1) / (2 * kCorrectWaves)));
+    // about two tiles per wave, at most two blocks per CU (one round of
+    // resident blocks: 512 on MI355X's 256 CUs)
+    const long want = std::max(1L, std::min(2L * compute_units(), (total + 2 * kCorrectWaves - 1) / (2 * kCorrectWaves)));
     bm.blk0[0] = 0;
     for (int f = 0; f < 6; ++f) {
         long nbf = bm.ntile[f] ? (want * bm.ntile[f] + total - 1) / total : 0;
