@@ -255,15 +255,27 @@ def cpu_baseline(w, budget_s=15.0):
     t0 = time.perf_counter()
     o.step(2)
     t1 = time.perf_counter() - t0
-    nsteps = int(max(2, min(1500, budget_s / max(t1, 1e-3))))
+    nsteps = int(max(2, min(1400, budget_s / max(t1, 1e-3))))  # the one-thread steps follow, below 1,600
     t0 = time.perf_counter()
     o.run(nsteps, it0=3)
     el = time.perf_counter() - t0
     cells = int(np.prod(w["shape"]))
+    # the same restatement on one thread, a few more steps (~5 s)
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    o.step(3 + nsteps)
+    t1 = time.perf_counter() - t0
+    n1 = int(max(1, min(100, 5.0 / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    o.run(n1, it0=4 + nsteps)
+    el1 = time.perf_counter() - t0
+    O.set_threads(cores)
     return {"value": round(cells * nsteps / el / 1e9, 4), "unit": "Gcells/s", "cores": cores,
             "kind": "port",
             "sample": f"{nsteps} steps of the same workload after 3 warm-up steps, f32, "
-                      f"{cores} OpenMP threads ({el:.1f} s)"}
+                      f"{cores} OpenMP threads ({el:.1f} s)",
+            "one_thread": {"value": round(cells * n1 / el1 / 1e9, 5), "unit": "Gcells/s", "cores": 1,
+                           "sample": f"{n1} further steps on one thread ({el1:.1f} s)"}}
 
 
 def gradient_workload(args):
