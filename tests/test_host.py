@@ -5,6 +5,8 @@ import re
 
 import numpy as np
 import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
 
 from oracle import oracle as O
 from seiswave_taichi_amd import _lib as L
@@ -404,6 +406,27 @@ def test_split_shell_layout_matches_the_sweeps(case):
     computes (the kernels' own index function, run on the host) cover every
     updated cell outside the box exactly once, with distinct slots inside
     each component's storage, at the slots the host-side unpacking reads."""
+    _check_split_shell_layout(case)
+
+
+@settings(max_examples=25, deadline=None)
+@given(st.data())
+def test_split_shell_layout_random_boxes(data):
+    """The same checks on random grid shapes, orders and split-free boxes."""
+    order = data.draw(st.integers(1, 4))
+    shape, ranges = [], []
+    for _ in range(3):
+        n = data.draw(st.integers(2 * order + 2, 24))
+        shape.append(n)
+        if data.draw(st.booleans()):
+            ranges.append((0, 0))  # no split-free range on this axis: all shell
+        else:
+            lo = data.draw(st.integers(0, n - 1))
+            ranges.append((lo, data.draw(st.integers(lo + 1, n))))
+    _check_split_shell_layout((tuple(shape), order, *ranges))
+
+
+def _check_split_shell_layout(case):
     from seiswave_taichi_amd._engine3d import SplitShell
     (nx, ny, nz), order, rx, ry, rz = case
     g = _fake_3d(order=order)
