@@ -8,7 +8,7 @@
 //
 // Execution: each thread owns one (i, j) column and marches along z over a
 // chunk of planes; a block takes 256 consecutive columns of a box in x-fastest
-// order.  On large grids a sweep is five launches over boxes of full-width
+// order, its waves starting on 128-byte lines.  On large grids a sweep is five launches over boxes of full-width
 // rows, instantiated with the CPML of only the axes whose slabs they meet (x;
 // x and y; x, y and z), so the bulk of the grid runs with fewer registers and
 // more waves in flight; on grids where the CPML shell is a large part, one
@@ -40,8 +40,15 @@ namespace sw {
 
 namespace {
 
+// Columns are numbered by their flat index q = j*nx + i within a plane,
+// from f0 (the box's first column rounded down to a multiple of 32) to fn:
+// a wave's 64 columns then start on a 128-byte line of every array (the
+// planes of a grid with nx*ny a multiple of 32 all start on a line), so a
+// row of 64 floats costs 2 line requests instead of 3.  Columns outside the
+// box in that range (the x edges of each row) are idle lanes.
 struct Box {
     int i0, j0, w, h, k0, k1, ncb;  // ncb: column blocks of the box
+    int f0, fn, nx;                 // flat column range [f0, fn)
 };
 
 struct Geo {
@@ -64,13 +71,14 @@ __device__ __forceinline__ Cell box_cell(const Geo &geo)
     const int gi = b / (kGroup * nzc), r = b - gi * kGroup * nzc;
     const int gw = min(kGroup, B.ncb - gi * kGroup);
     const int zb = r / gw, cb = gi * kGroup + r - zb * gw;
-    const int t = cb * kThreads + (int)threadIdx.x;
+    const int q = B.f0 + cb * kThreads + (int)threadIdx.x;
+    const int j = q / B.nx, i = q - j * B.nx;
     Cell c;
-    if (t >= B.w * B.h) {
+    if (q >= B.fn || j < B.j0 || j >= B.j0 + B.h || i < B.i0 || i >= B.i0 + B.w) {
         c.i = -1; c.j = -1;
     } else {
-        c.i = B.i0 + t % B.w;
-        c.j = B.j0 + t / B.w;
+        c.i = i;
+        c.j = j;
     }
     c.k0 = B.k0 + zb * kZChunk;
     c.k1 = min(c.k0 + kZChunk, B.k1);
@@ -436,7 +444,7 @@ struct Plan {
 // 4.74 / 5.08 (5.03 with 64-plane chunks); in groups of 32 column blocks
 // with chunks of 2 / 4 / 6 / 8 planes, 200^3 0.309 / 0.305 / 0.313 / 0.317,
 // 600x600x400 4.77 (4) / 4.92 (8).
-void add_box(Plan &p, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
+void add_box(Plan &p, int nx, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
 {
     if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
     Geo &geo = p.g[p.n];
@@ -445,7 +453,10 @@ void add_box(Plan &p, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
     b.i0 = i0; b.w = i1 - i0;
     b.j0 = j0; b.h = j1 - j0;
     b.k0 = k0; b.k1 = k1;
-    b.ncb = (b.w * b.h + kThreads - 1) / kThreads;
+    b.nx = nx;
+    b.f0 = (j0 * nx + i0) & ~31;
+    b.fn = (j1 - 1) * nx + i1;
+    b.ncb = (b.fn - b.f0 + kThreads - 1) / kThreads;
     geo.nblock = (int)((long)b.ncb * ((k1 - k0 + kZChunk - 1) / kZChunk));
 }
 
@@ -461,13 +472,13 @@ Plan make_plan(const sw_ssg3d &g, int N)
     // 600x600x400 grid, bulk 84 %, 5 % faster split).
     const double bulk = (yl < yh && zl < zh) ? (double)(yh - yl) * (zh - zl) / ((double)(y1 - y0) * (z1 - z0)) : 0.0;
     if (bulk >= 0.75) {
-        add_box(p, 1, x0, x1, yl, yh, zl, zh);
-        add_box(p, 3, x0, x1, y0, yl, zl, zh);
-        add_box(p, 3, x0, x1, yh, y1, zl, zh);
-        add_box(p, 7, x0, x1, y0, y1, z0, zl);
-        add_box(p, 7, x0, x1, y0, y1, zh, z1);
+        add_box(p, g.nx, 1, x0, x1, yl, yh, zl, zh);
+        add_box(p, g.nx, 3, x0, x1, y0, yl, zl, zh);
+        add_box(p, g.nx, 3, x0, x1, yh, y1, zl, zh);
+        add_box(p, g.nx, 7, x0, x1, y0, y1, z0, zl);
+        add_box(p, g.nx, 7, x0, x1, y0, y1, zh, z1);
     } else {
-        add_box(p, 7, x0, x1, y0, y1, z0, z1);
+        add_box(p, g.nx, 7, x0, x1, y0, y1, z0, z1);
     }
     return p;
 }
@@ -522,7 +533,7 @@ Plan make_interior_plan(const sw_ssg3d &g, int N)
 {
     Plan p;
     const SplitBox b = split_box(g, N);
-    add_box(p, 0, b.xl, b.xh, b.yl, b.yh, b.zl, b.zh);
+    add_box(p, g.nx, 0, b.xl, b.xh, b.yl, b.yh, b.zl, b.zh);
     return p;
 }
 

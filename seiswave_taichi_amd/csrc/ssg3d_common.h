@@ -12,8 +12,10 @@ namespace {
 
 // A launch covers one box of cells: columns [i0, i0+w) x [j0, j0+h) over
 // planes [k0, k1).  The box is cut into blocks of kThreads consecutive
-// columns (x fastest, so no lane idles on a ragged x edge and a wave reads at
-// most two row segments per tap) times chunks of kZChunk planes.
+// columns in x-fastest order times chunks of kZChunk planes.  The plain
+// sweeps count columns by flat index j*nx + i from a 128-byte line boundary
+// (lanes on the x edges of a row idle); the split-field shell counts only
+// the shell's own columns.
 constexpr int kThreads = 256;
 
 // Blocks run in groups of kGroup column blocks (8,192 columns) that march
