@@ -57,6 +57,12 @@ struct SBox {
     int gi, gwi, gj, gwj;
     int si, di, sj, dj, oj;
     uint32_t C, pitch, kstride;
+    // boxes of full rows (no column gap) number their columns by flat index
+    // v = j*nx + i over rows [j0, j0+h) before the row gap, from f0 (rounded
+    // down to a multiple of 64) to fn: a wave's columns then start on a
+    // 128-byte line of every field and, in the z and y slabs, on a 64-slot
+    // block of the split storage; lanes on the x edges of a row idle
+    int flat, nx, f0, fn;
 };
 
 struct ShellGeo {
@@ -89,12 +95,23 @@ __host__ __device__ inline SCell shell_cell(const ShellGeo &geo, int b, int t)
     SCell o;
     o.k0 = B.k0 + zb * kZChunk;
     o.k1 = o.k0 + kZChunk < B.k1 ? o.k0 + kZChunk : B.k1;
-    if (c >= B.w * B.h) {
+    int i, j;
+    bool idle;
+    if (B.flat) {
+        const int v = B.f0 + c;
+        j = v / B.nx;
+        i = v - j * B.nx;
+        idle = v >= B.fn || j < B.j0 || j >= B.j0 + B.h || i < B.i0 || i >= B.i0 + B.w;
+    } else {
+        idle = c >= B.w * B.h;
+        i = B.i0 + c % B.w;
+        j = B.j0 + c / B.w;
+        if (i >= B.gi) i += B.gwi;
+    }
+    if (idle) {
         o.i = -1; o.j = -1; o.q0 = 0; o.qk = 0;
         return o;
     }
-    int i = B.i0 + c % B.w, j = B.j0 + c / B.w;
-    if (i >= B.gi) i += B.gwi;
     if (j >= B.gj) j += B.gwj;
     o.i = i;
     o.j = j;
@@ -124,7 +141,11 @@ ShellGeo make_shell_geo(const sw_ssg3d &g, int N, int part)
         B.i0 = x0; B.w = w; B.gi = gwi ? gi : INT_MAX; B.gwi = gwi;
         B.j0 = y0; B.h = h; B.gj = gwj ? gj : INT_MAX; B.gwj = gwj;
         B.k0 = k0; B.k1 = k1;
-        B.ncb = (w * h + kThreads - 1) / kThreads;
+        B.flat = gwi == 0;
+        B.nx = g.nx;
+        B.f0 = (y0 * g.nx + x0) & ~63;
+        B.fn = (y0 + h - 1) * g.nx + x0 + w;
+        B.ncb = B.flat ? (B.fn - B.f0 + kThreads - 1) / kThreads : (w * h + kThreads - 1) / kThreads;
         geo.start[geo.nbox++] = nblock;
         nblock += B.ncb * ((k1 - k0 + kZChunk - 1) / kZChunk);
     };
