@@ -128,14 +128,17 @@ __global__ void k_band_gather(Fields9 F, int nx, int ny, BandGeo q, float *out)
 //    9 g + 4 nb + r (zero weights where 4 nb + r >= 9).
 // LDS holds only the face's permuted weights (52 KB), so no barrier follows
 // the one after staging them; while one wave issues matrix products the
-// other waves of its SIMD gather, evaluate tanh or scatter.  Blocks of 10
-// waves at 5 waves per SIMD (96 VGPRs: the next tile is not prefetched) put
-// 5,120 waves on MI355X's 256 CUs, so the 8,553 tiles of the 200^3 case
-// take at most two per wave (at 4 waves per SIMD, 3 for some: measured 62.7
-// against 58.7 us).  Each block serves one face: the host gives every face a
-// share of the blocks in proportion to its tiles.
+// other waves of its SIMD gather, evaluate tanh or scatter.  A wave gathers
+// its next tile while it computes the current one, and its first tile while
+// the block stages the weights (110 VGPRs).  One block of 12 waves per CU
+// (3 per SIMD; about three tiles per wave: the 8,553 tiles of the 200^3 case
+// on 238 blocks) keeps the gathers behind the matrix products.  Measured on
+// MI355X at 200^3: 53.8 us per launch, against 59.3 us for 10-wave blocks at
+// 5 waves per SIMD without the prefetch (at most two tiles per wave) and
+// 82 us for 8- or 16-wave blocks with it.  Each block serves one face: the
+// host gives every face a share of the blocks in proportion to its tiles.
 constexpr int kWaveCols = 16;
-constexpr int kCorrectWaves = 10;
+constexpr int kCorrectWaves = 12;
 constexpr int kCorrectThreads = 64 * kCorrectWaves;
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
@@ -209,7 +212,7 @@ __device__ __forceinline__ void transpose4(float *h)
     h[3] = __uint_as_float(t23[1]);
 }
 
-__global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_per_eu(5))) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, BlockMap bm,
+__global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_per_eu(3))) k_band_correct(Fields9 F, int nx, int ny, BandGeo q, BlockMap bm,
                                                                   const float *params, float *prev)
 {
     __shared__ __attribute__((aligned(16))) float w[kStaged];
@@ -220,6 +223,11 @@ __global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_p
     const int nb_f = bm.blk0[f + 1] - bm.blk0[f], lb = b - bm.blk0[f];
     const int t0 = (int)(((long)lb * bm.ntile[f]) / nb_f);
     const int t1 = (int)(((long)(lb + 1) * bm.ntile[f]) / nb_f);
+    const int wv = (int)threadIdx.x / 64, ln = (int)threadIdx.x % 64;
+    const int li = ln % 16, lg = ln / 16;
+    BandIO io;
+    int tile = t0 + wv;
+    if (tile < t1) fetch_band(io, F, nx, ny, q, f, prev, tile, li, lg);
     {
         // All of a thread's loads are issued before its first LDS store, so
         // the ~26 L2 reads per thread overlap instead of running one by one.
@@ -264,11 +272,10 @@ __global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_p
     __syncthreads();
     const float *W1P = w + kSW1, *B1 = w + kSB1, *W2P = w + kSW2, *B2 = w + kSB2, *W3P = w + kSW3;
     const float *B3P = w + kSB3, *OSP = w + kSOS, *ISI = w + kSIS;
-    const int wv = (int)threadIdx.x / 64, ln = (int)threadIdx.x % 64;
-    const int li = ln % 16, lg = ln / 16;
-    BandIO io;
-    for (int tile = t0 + wv; tile < t1; tile += kCorrectWaves) {
-        fetch_band(io, F, nx, ny, q, f, prev, tile, li, lg);
+    for (; tile < t1; tile += kCorrectWaves) {
+        BandIO nio;
+        const bool more = tile + kCorrectWaves < t1;
+        if (more) fetch_band(nio, F, nx, ny, q, f, prev, tile + kCorrectWaves, li, lg);
         // the staged weights are read where they are used: without this the
         // compiler hoists all of them out of the tile loop into registers
         asm volatile("" ::: "memory");
@@ -356,6 +363,7 @@ __global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_p
                 prev[(long)io.col * kOut + lg * kF + u] = v;
             }
         }
+        if (more) io = nio;
     }
 }
 
@@ -484,9 +492,9 @@ int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params
         total += bm.ntile[f];
     }
     if (total == 0) return SW_OK;
-    // about two tiles per wave, at most two blocks per CU (one round of
-    // resident blocks: 512 on MI355X's 256 CUs)
-    const long want = std::max(1L, std::min(2L * compute_units(), (total + 2 * kCorrectWaves - 1) / (2 * kCorrectWaves)));
+    // about three tiles per wave, at most one block per CU (one round of
+    // resident blocks: at most 256 on MI355X)
+    const long want = std::max(1L, std::min((long)compute_units(), (total + 3 * kCorrectWaves - 1) / (3 * kCorrectWaves)));
     bm.blk0[0] = 0;
     for (int f = 0; f < 6; ++f) {
         long nbf = bm.ntile[f] ? (want * bm.ntile[f] + total - 1) / total : 0;
