@@ -117,11 +117,15 @@ def test_band_geometry_and_gather():
     assert (np.minimum(full, ext - 1 - full).min(1) < 4).sum() == ncol * 4
 
 
-def test_fused_correction_matches_oracle():
+# 20: every face a whole number of 16-column tiles, one block per face;
+# 23: ragged last tiles; 61: about three tiles per wave (each wave gathers its
+# next tile while it computes the current one)
+@pytest.mark.parametrize("n_int", [20, 23, 61])
+def test_fused_correction_matches_oracle(n_int):
     from seiswave_taichi_amd import _lib as L
     from seiswave_taichi_amd import learned_abc as LA
     lib = LA._sig()
-    w = _small_sim()
+    w = _small_sim(n_int)
     e = w._e
     g = torch.Generator(device="cuda").manual_seed(0)
     for nm in e.f:
