@@ -153,6 +153,21 @@ def traffic_from_profile(kernel_tag, workload_name):
         return json.load(fh).get("bytes_per_launch")
 
 
+def copy_calibration(gib):
+    """Three device copies of a ``gib`` GiB buffer: a kernel of known traffic
+    (reads and writes ``gib`` GiB each) by which rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters are scaled to bytes.  Returns the buffer's bytes."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src_buf = torch.zeros(n, dtype=torch.float32, device="cuda")
+    dst_buf = torch.empty_like(src_buf)
+    for _ in range(3):
+        dst_buf.copy_(src_buf)
+    torch.cuda.synchronize()
+    del src_buf, dst_buf
+    return n * 4
+
+
 def copy_ceiling_gbs(gib=1.0, reps=5):
     """Read + write rate of a device-to-device copy of `gib` GiB (bytes moved /
     time, HIP events on the copy's stream): the rate a pure streaming kernel
@@ -524,6 +539,10 @@ def learned_step_workload(args):
     flops = 2.0 * ncol * (72 * 64 + 64 * 64 + 64 * 36)
     flops_mfma = 2.0 * ncol * (72 * 64 + 64 * 64 + 64 * 48)
     ach = flops / (k_ms / 1e3) / 1e12
+    # per column: the 36 band values and the 36 of the previous step read,
+    # the 36 corrected values written to the fields and kept for the next step
+    mem = 4 * 4 * 36 * ncol
+    traffic = traffic_from_profile("band_correct", "learned_200_step")
     out = {"metric": "learned boundary step cost", "value": round(ms["wide_20_cell"] / ms["thin_4_cell_plus_network"], 4),
            "unit": "x (speedup of the learned step over the 20-cell CPML step)", "higher_is_better": True,
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
@@ -535,7 +554,12 @@ def learned_step_workload(args):
            "roofline": {"bound": "mfma", "kernel": "k_band_correct", "achieved": round(ach, 2),
                         "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / MFMA_F32_PEAK_TFLOPS, 4),
                         "flops_per_launch": flops, "mfma_flops_per_launch": flops_mfma,
-                        "launch_ms": round(k_ms, 5), "traffic": None}}
+                        "launch_ms": round(k_ms, 5), "traffic": traffic},
+           "memory": {"algorithmic_bytes_per_launch": mem, "GBs": round(mem / (k_ms / 1e3) / 1e9, 1),
+                      "traffic_GBs": round(traffic / (k_ms / 1e3) / 1e9, 1) if traffic else None,
+                      "frac_of_hbm_peak": round((traffic or mem) / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    if args.copy_calibration > 0:
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     probe = mfma_ceiling_tflops()
     out["roofline"]["mfma_probe_TFLOPs"] = probe["TFLOPs"]
     out["roofline"]["frac_of_mfma_probe"] = round(flops_mfma / (k_ms / 1e3) / 1e12 / probe["TFLOPs"], 4)
@@ -689,14 +713,7 @@ def main():
         rs["stream_ceiling_GBs"] = blended_ceiling(streams, b_split, n_u, dom)
         rs["frac_of_stream_ceiling"] = round(rs["achieved"] / rs["stream_ceiling_GBs"], 4)
     if args.copy_calibration > 0:
-        n = int(args.copy_calibration * (1 << 30)) // 4
-        src_buf = torch.zeros(n, dtype=torch.float32, device="cuda")
-        dst_buf = torch.empty_like(src_buf)
-        for _ in range(3):
-            dst_buf.copy_(src_buf)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = n * 4
-        del src_buf, dst_buf
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
