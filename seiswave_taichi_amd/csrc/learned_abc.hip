@@ -182,20 +182,24 @@ struct BandIO {
 __device__ __forceinline__ void fetch_band(BandIO &b, const Fields9 &F, int nx, int ny, const BandGeo &q, int f,
                                            const float *prev, int tile, int li, int lg)
 {
-    const int r = tile * kWaveCols + li;  // column within the face
-    b.live = r < q.ncol[f];
+    const int r0 = tile * kWaveCols + li;  // column within the face
+    b.live = r0 < q.ncol[f];
+    const int r = b.live ? r0 : q.ncol[f] - 1;  // a dead lane reads the face's last column
     b.col = q.col0[f] + r;
-    b.pc = 0;
-    if (b.live) {
-        const int na = lateral_a_extent(q, f);
-        int i, j, k;
-        band_cell(q, f, r % na, r / na, lg, i, j, k);
-        b.pc = ((long)k * ny + j) * nx + i;
+    const int na = lateral_a_extent(q, f);
+    int i, j, k;
+    band_cell(q, f, r % na, r / na, lg, i, j, k);
+    b.pc = ((long)k * ny + j) * nx + i;
+    float cv[kF], pv[kF];
+#pragma unroll
+    for (int u = 0; u < kF; ++u) {
+        cv[u] = F.f[u][b.pc];
+        pv[u] = prev[(long)b.col * kOut + lg * kF + u];
     }
 #pragma unroll
     for (int u = 0; u < kF; ++u) {
-        b.cur[u] = b.live ? F.f[u][b.pc] : 0.f;
-        b.pv[u] = b.live ? prev[(long)b.col * kOut + lg * kF + u] : 0.f;
+        b.cur[u] = b.live ? cv[u] : 0.f;
+        b.pv[u] = b.live ? pv[u] : 0.f;
     }
 }
 
