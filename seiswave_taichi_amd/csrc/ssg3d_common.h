@@ -28,7 +28,9 @@ constexpr int kThreads = 256;
 // plane-by-plane order; 600x600x400 4.77 (32), 4.73 (64), 4.91 plane by plane.
 // With line-aligned columns, groups of 16 / 32 / 64: 200^3 0.301 / 0.289 /
 // 0.290, 600x600x400 4.57 / 4.42 / 4.38, 200^3 split-field class 0.431 /
-// 0.425 / 0.422.
+// 0.425 / 0.422; groups of 32 / 64 / 128 / 256 on 600x600x400 4.429 /
+// 4.385 / 4.374 / 4.414 and on its 600x600x54 shard 0.577 / 0.581 / 0.577 /
+// 0.589 (one group size for every grid: the gains elsewhere stay near 1 %).
 constexpr int kGroup = 32;
 
 // Fields are addressed by 32-bit byte offsets from their base pointers: one
