@@ -158,9 +158,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
                 st(R.mz, 0 * sl.nzs + oz, P.mz[0]);
             }
             if (CLS == SW_CLASS_SPML)  // src/Elastic3D_SSG_SPML.py:203: x+y+z, times 1/rho
-                st(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * (1.f / P.rho[0]));
+                st_nt(R.vx, p, P.v[0] + (dsxx + dsxy + dsxz) * dt * (1.f / P.rho[0]));
             else
-                st(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
+                st_nt(R.vx, p, P.v[0] + (dsxx + dsxz + dsxy) * dt / P.rho[0]);
         }
         // vy at (i, j+1/2, k): :180-195
         if (vy_col && k >= N + 1) {
@@ -181,9 +181,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
                 st(R.mz, 1 * sl.nzs + oz, P.mz[1]);
             }
             if (CLS == SW_CLASS_SPML)  // :231
-                st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * (1.f / rho_half_y));
+                st_nt(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt * (1.f / rho_half_y));
             else
-                st(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
+                st_nt(R.vy, p, P.v[1] + (dsxy + dsyy + dsyz) * dt / rho_half_y);
         }
         // vz at (i, j, k+1/2): :197-212 (Tensor :228-243)
         if (vz_col && k < nz - N - 1) {
@@ -204,11 +204,11 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
                 st(R.mz, 2 * sl.nzs + oz, P.mz[2]);
             }
             if (CLS == SW_CLASS_SPML)  // :257
-                st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * (1.f / rho_avg));
+                st_nt(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt * (1.f / rho_avg));
             else if (CLS == SW_CLASS_TENSOR)
-                st(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt / rho_avg);
+                st_nt(R.vz, p, P.v[2] + (dsxz + dsyz + dszz) * dt / rho_avg);
             else
-                st(R.vz, p, P.v[2] + (dszz + dsxz + dsyz) * dt / rho_avg);
+                st_nt(R.vz, p, P.v[2] + (dszz + dsxz + dsyz) * dt / rho_avg);
         }
         win_shift<N>(wxz);
         win_shift<N>(wyz);
@@ -259,12 +259,12 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
     P.mu[1] = ldu(R.mu, vb, c0 + 4);
     P.mu[2] = ldu(R.mu, vb, c0 + SY);
     P.mu[3] = ldu(R.mu, vb, c0 + (int)SZ);
-    P.s[0] = ldu(R.sxx, vb, c0);
-    P.s[1] = ldu(R.syy, vb, c0);
-    P.s[2] = ldu(R.szz, vb, c0);
-    P.s[3] = ldu(R.sxy, vb, c0);
-    P.s[4] = ldu(R.sxz, vb, c0);
-    P.s[5] = ldu(R.syz, vb, c0);
+    P.s[0] = ldu_nt(R.sxx, vb, c0);
+    P.s[1] = ldu_nt(R.syy, vb, c0);
+    P.s[2] = ldu_nt(R.szz, vb, c0);
+    P.s[3] = ldu_nt(R.sxy, vb, c0);
+    P.s[4] = ldu_nt(R.sxz, vb, c0);
+    P.s[5] = ldu_nt(R.syz, vb, c0);
     P.wvx = ldu(R.vx, vb, c0 + N * (int)SZ);
     P.wvy = ldu(R.vy, vb, c0 + N * (int)SZ);
     P.wvz = ldu(R.vz, vb, c0 + (N - 1) * (int)SZ);
@@ -357,9 +357,9 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvzz = cpml_step(dvzz, P.mz[0], P.cz.a, P.cz.b, P.cz.k);
                 st(R.mz, 3 * sl.nzs + oz, P.mz[0]);
             }
-            st(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
-            st(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
-            st(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
+            st_nt(R.sxx, p, P.s[0] + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
+            st_nt(R.syy, p, P.s[1] + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
+            st_nt(R.szz, p, P.s[2] + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
         }
         // sxy at (i+1/2, j+1/2, k): :267-279
         if (sxy_col) {
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvxy = cpml_step(dvxy, P.my[1], cy.ah, cy.bh, cy.kh);
                 st(R.my, 4 * sl.nys + oy, P.my[1]);
             }
-            st(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
+            st_nt(R.sxy, p, P.s[3] + mu_a * (dvxy + dvyx) * dt);
         }
         // sxz at (i+1/2, j, k+1/2): :281-293
         if (sxz_col && k < nz - N - 1) {
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvxz = cpml_step(dvxz, P.mz[1], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 4 * sl.nzs + oz, P.mz[1]);
             }
-            st(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
+            st_nt(R.sxz, p, P.s[4] + mu_a * (dvxz + dvzx) * dt);
         }
         // syz at (i, j+1/2, k+1/2): :295-307
         if (syz_col && k < nz - N - 1) {
@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
                 dvyz = cpml_step(dvyz, P.mz[2], P.cz.ah, P.cz.bh, P.cz.kh);
                 st(R.mz, 5 * sl.nzs + oz, P.mz[2]);
             }
-            st(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
+            st_nt(R.syz, p, P.s[5] + mu_a * (dvyz + dvzy) * dt);
         }
         win_shift<N>(wvz);
         win_shift<N>(wvx);

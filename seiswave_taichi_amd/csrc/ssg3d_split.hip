@@ -338,9 +338,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity_shell(sw_ssg3d g, ShellGe
                 const float a = cn(qq[0], 1.f - hdt * dxh, 1.f + hdt * dxh, dsxx * dt * rho_over);
                 const float b = cn(qq[1], 1.f - hdt * dyi, 1.f + hdt * dyi, dsxy * dt * rho_over);
                 const float e = cn(qq[2], 1.f - hdt * dzi, 1.f + hdt * dzi, dsxz * dt * rho_over);
-                st(R.vx, p, a + b + e);
+                st_nt(R.vx, p, a + b + e);
             } else {
-                st(R.vx, p, (MID ? P.v[0] : ld(R.vx, p)) + (dsxx + dsxy + dsxz) * dt * rho_over);
+                st_nt(R.vx, p, (MID ? P.v[0] : ld(R.vx, p)) + (dsxx + dsxy + dsxz) * dt * rho_over);
             }
         }
         // vy at (i, j+1/2, k): :209-233
@@ -354,9 +354,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity_shell(sw_ssg3d g, ShellGe
                 const float a = cn(qq[3], 1.f - hdt * dxi, 1.f + hdt * dxi, dt * dsxy * rho_over);
                 const float b = cn(qq[4], 1.f - hdt * dyh, 1.f + hdt * dyh, dt * dsyy * rho_over);
                 const float e = cn(qq[5], 1.f - hdt * dzi, 1.f + hdt * dzi, dt * dsyz * rho_over);
-                st(R.vy, p, a + b + e);
+                st_nt(R.vy, p, a + b + e);
             } else {
-                st(R.vy, p, (MID ? P.v[1] : ld(R.vy, p)) + (dsxy + dsyy + dsyz) * dt * rho_over);
+                st_nt(R.vy, p, (MID ? P.v[1] : ld(R.vy, p)) + (dsxy + dsyy + dsyz) * dt * rho_over);
             }
         }
         // vz at (i, j, k+1/2): :235-259
@@ -370,9 +370,9 @@ __global__ void __launch_bounds__(kThreads) k_velocity_shell(sw_ssg3d g, ShellGe
                 const float a = cn(qq[6], 1.f - hdt * dxi, 1.f + hdt * dxi, dt * dsxz * rho_over);
                 const float b = cn(qq[7], 1.f - hdt * dyi, 1.f + hdt * dyi, dt * dsyz * rho_over);
                 const float e = cn(qq[8], 1.f - hdt * dzh, 1.f + hdt * dzh, dt * dszz * rho_over);
-                st(R.vz, p, a + b + e);
+                st_nt(R.vz, p, a + b + e);
             } else {
-                st(R.vz, p, (MID ? P.v[2] : ld(R.vz, p)) + (dsxz + dsyz + dszz) * dt * rho_over);
+                st_nt(R.vz, p, (MID ? P.v[2] : ld(R.vz, p)) + (dsxz + dsyz + dszz) * dt * rho_over);
             }
         }
         // the split groups, updated or not
@@ -506,13 +506,13 @@ __global__ void __launch_bounds__(kThreads) k_stress_shell(sw_ssg3d g, ShellGeo 
                 const float c0 = cn(PQ.q[12], xn, xd, dt * lam_h * dvxx);
                 const float c1 = cn(PQ.q[13], yn, yd, dt * lam_h * dvyy);
                 const float c2 = cn(PQ.q[14], zn, zd, dt * lpm * dvzz);
-                st(R.sxx, p, a0 + a1 + a2);
-                st(R.syy, p, b0 + b1 + b2);
-                st(R.szz, p, c0 + c1 + c2);
+                st_nt(R.sxx, p, a0 + a1 + a2);
+                st_nt(R.syy, p, b0 + b1 + b2);
+                st_nt(R.szz, p, c0 + c1 + c2);
             } else {
-                st(R.sxx, p, ld(R.sxx, p) + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
-                st(R.syy, p, ld(R.syy, p) + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
-                st(R.szz, p, ld(R.szz, p) + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
+                st_nt(R.sxx, p, ld(R.sxx, p) + (lpm * dvxx + lam_h * (dvzz + dvyy)) * dt);
+                st_nt(R.syy, p, ld(R.syy, p) + (lpm * dvyy + lam_h * (dvxx + dvzz)) * dt);
+                st_nt(R.szz, p, ld(R.szz, p) + (lpm * dvzz + lam_h * (dvxx + dvyy)) * dt);
             }
         }
         // sxy at (i+1/2, j+1/2, k): :334-350
@@ -523,9 +523,9 @@ __global__ void __launch_bounds__(kThreads) k_stress_shell(sw_ssg3d g, ShellGeo 
             if (xh | yh | zi) {
                 const float a = cn(PQ.q[6], 1.f - hdt * dxh, 1.f + hdt * dxh, dt * mu_a * dvyx);
                 const float b = cn(PQ.q[7], 1.f - hdt * dyh, 1.f + hdt * dyh, dt * mu_a * dvxy);
-                st(R.sxy, p, a + b);
+                st_nt(R.sxy, p, a + b);
             } else {
-                st(R.sxy, p, ld(R.sxy, p) + mu_a * (dvxy + dvyx) * dt);
+                st_nt(R.sxy, p, ld(R.sxy, p) + mu_a * (dvxy + dvyx) * dt);
             }
         }
         // sxz at (i+1/2, j, k+1/2): :352-371
@@ -536,9 +536,9 @@ __global__ void __launch_bounds__(kThreads) k_stress_shell(sw_ssg3d g, ShellGeo 
             if (xh | yi | zh) {
                 const float a = cn(PQ.q[8], 1.f - hdt * dxh, 1.f + hdt * dxh, dt * mu_a * dvzx);
                 const float b = cn(PQ.q[9], 1.f - hdt * dzh, 1.f + hdt * dzh, dt * mu_a * dvxz);
-                st(R.sxz, p, a + b);
+                st_nt(R.sxz, p, a + b);
             } else {
-                st(R.sxz, p, ld(R.sxz, p) + mu_a * (dvxz + dvzx) * dt);
+                st_nt(R.sxz, p, ld(R.sxz, p) + mu_a * (dvxz + dvzx) * dt);
             }
         }
         // syz at (i, j+1/2, k+1/2): :373-391
@@ -549,9 +549,9 @@ __global__ void __launch_bounds__(kThreads) k_stress_shell(sw_ssg3d g, ShellGeo 
             if (xi | yh | zh) {
                 const float a = cn(PQ.q[10], 1.f - hdt * dyh, 1.f + hdt * dyh, dt * mu_a * dvzy);
                 const float b = cn(PQ.q[11], 1.f - hdt * dzh, 1.f + hdt * dzh, dt * mu_a * dvyz);
-                st(R.syz, p, a + b);
+                st_nt(R.syz, p, a + b);
             } else {
-                st(R.syz, p, ld(R.syz, p) + mu_a * (dvyz + dvzy) * dt);
+                st_nt(R.syz, p, ld(R.syz, p) + mu_a * (dvyz + dvzy) * dt);
             }
         }
         st4(sq.blk + 576 + 4 * sq.l, PQ.q);
