@@ -10,9 +10,11 @@ tensor (10 Hz Ricker) at the centre, 50x50 receivers at k = s+5; 1500 steps.
 
 Prints one JSON line: Gcells/s (nx*ny*nz*steps/t), the roofline of the
 dominant kernel (algorithmic HBM bytes per launch / measured launch time
-against 8 TB/s, and against the measured read + write rate of a 1 GiB
-device copy, the practical ceiling of a streaming kernel) and a CPU baseline (the oracle's C restatement with OpenMP on
-this host's cores, on a bounded sample of the same workload).
+against 8 TB/s; against the library's streaming probe with the sweeps' array
+mixes and non-temporal accesses, the practical ceiling of a streaming kernel;
+and against the read + write rate of a 1 GiB device copy) and a CPU baseline
+(the oracle's C restatement with OpenMP on this host's cores, and on one
+thread, on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
