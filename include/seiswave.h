@@ -166,11 +166,23 @@ int64_t sw_ssg3d_split_cells(const sw_ssg3d *g);
 int64_t sw_ssg3d_split_map(const sw_ssg3d *g, int32_t sweep, int64_t *cell, int64_t *slot, int64_t max);
 /* Launch plan of the two 3-D sweeps for this descriptor (no counterpart in
  * the reference, whose loops are one Taichi task each): returns the number of
- * launches per sweep (1, or 5 when the CPML-free bulk is >= 75 % of the
- * updated cells) and writes up to max CPML axis masks (bit 0 x, 1 y, 2 z:
- * the axes whose CPML or split-field code the launch instantiates).  Host
- * only: no device access. */
+ * launches per sweep (1, or 5: the CPML-free bulk with the x CPML only, the
+ * y slabs, the z slabs) and writes up to max CPML axis masks (bit 0 x, 1 y,
+ * 2 z: the axes whose CPML or split-field code the launch instantiates).
+ * Host only: no device access. */
 int sw_ssg3d_launch_plan(const sw_ssg3d *g, int32_t *pm, int32_t max);
+/* How the 3-D sweeps choose their launch plan, process-wide (no counterpart
+ * in the reference).  SW_PLAN_AUTO (the default): five launches when the
+ * CPML-free bulk is >= 75 % of the updated cells and every slab launch has
+ * at least as many blocks as the device has compute units, else one.
+ * SW_PLAN_ONE: always one launch.  SW_PLAN_BOXES: five launches whenever the
+ * bulk box is not empty (for tests of the per-box kernels and for tuning).
+ * Returns the previous mode, or SW_EINVAL for another value.  Calls that
+ * overlap with running sweeps on other threads race. */
+#define SW_PLAN_AUTO 0
+#define SW_PLAN_ONE 1
+#define SW_PLAN_BOXES 2
+int sw_set_launch_plan(int32_t mode);
 /* Source injection for step it (the serial statements of the reference kernels). */
 int sw_ssg3d_inject(const sw_ssg3d *g, const sw_source *src, int32_t it, void *stream);
 /* Receiver gather for step it (src/Elastic3D_SSG_CPML.py:309, Tensor :344-345). */

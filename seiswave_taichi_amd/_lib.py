@@ -86,6 +86,7 @@ SIGNATURES = {
     "sw_ssg3d_velocity_spml": [_P3, C.c_void_p],
     "sw_ssg3d_stress_spml": [_P3, C.c_void_p],
     "sw_ssg3d_launch_plan": [_P3, C.POINTER(C.c_int32), C.c_int32],
+    "sw_set_launch_plan": [C.c_int32],
     "sw_ssg3d_split_cells": [_P3],
     "sw_ssg3d_split_map": [_P3, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64],
     "sw_ssg3d_inject": [_P3, _PS, C.c_int32, C.c_void_p],

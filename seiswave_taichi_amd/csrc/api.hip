@@ -2,14 +2,36 @@
 // gather, multi-step drivers and the free-surface helpers.
 #include "sw_internal.h"
 
+#include <atomic>
 #include <climits>
 #include <cstring>
+#include <mutex>
 
 namespace sw {
 
 namespace {
 thread_local char g_err[512] = "";
+std::atomic<int> g_plan_mode{SW_PLAN_AUTO};
 }
+
+int compute_units()
+{
+    static std::mutex mu;
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();  // no device here: clear the sticky error
+        return 256;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (cus[dev] == 0) {
+        int v = 0;
+        cus[dev] = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0 ? v : 256;
+    }
+    return cus[dev];
+}
+
+int launch_plan_mode() { return g_plan_mode.load(std::memory_order_relaxed); }
 
 void set_error(const char *fmt, ...)
 {
@@ -461,6 +483,13 @@ int sw_ssg3d_stress_spml(const sw_ssg3d *g, void *stream)
     if (rc) return rc;
     SW_REQUIRE(g->cls == SW_CLASS_SPML, "split-field stress sweep needs a split-field descriptor");
     return launch_ssg3d_stress_spml(g, as_stream(stream));
+}
+
+int sw_set_launch_plan(int32_t mode)
+{
+    SW_REQUIRE(mode == SW_PLAN_AUTO || mode == SW_PLAN_ONE || mode == SW_PLAN_BOXES, "unknown launch plan mode %d",
+               mode);
+    return sw::g_plan_mode.exchange(mode);
 }
 
 int sw_ssg3d_launch_plan(const sw_ssg3d *g, int32_t *pm, int32_t max)

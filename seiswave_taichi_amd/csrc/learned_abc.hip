@@ -470,22 +470,6 @@ int launch_band_gather(const sw_ssg3d *g, const sw_band *b, float *out, hipStrea
     return launch_status("band gather");
 }
 
-// Compute units of the current device, looked up once per device (256 on
-// MI355X; 256 if the query fails).
-static int compute_units()
-{
-    static std::mutex mu;
-    static int cus[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    std::lock_guard<std::mutex> lock(mu);
-    if (cus[dev] == 0) {
-        int v = 0;
-        cus[dev] = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0 ? v : 256;
-    }
-    return cus[dev];
-}
-
 int launch_band_correct(const sw_ssg3d *g, const sw_band *b, const float *params, float *prev, hipStream_t st)
 {
     const BandGeo q = band_geo(*b);

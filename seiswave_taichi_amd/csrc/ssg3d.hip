@@ -468,20 +468,29 @@ Plan make_plan(const sw_ssg3d &g, int N)
     const int x0 = N, x1 = g.nx - N, y0 = N, y1 = g.ny - N, z0 = N, z1 = g.nz - N;
     const int yl = std::max(y0, g.py.lo), yh = std::min(y1, g.py.hi);
     const int zl = std::max(z0, g.pz.lo), zh = std::min(z1, g.pz.hi);
-    // Several launches pay off only when the bulk box dominates: each extra
-    // launch adds a ramp-up and a tail (measured on MI355X: a 200^3 grid with
-    // 20-cell CPML, bulk 64 % of the cells, runs faster as one launch; the
-    // 600x600x400 grid, bulk 84 %, 5 % faster split).
+    // Several launches pay off only when the bulk box dominates and every
+    // slab launch fills the GPU: each extra launch adds a ramp-up and a tail,
+    // and a launch of fewer blocks than compute units leaves CUs idle for its
+    // whole length.  Measured on MI355X, ms per step: 200^3 with a 20-cell
+    // CPML (bulk 64 %) one launch 0.289 against 0.355 in five; 600x600x400
+    // (bulk 84 %, slab launches of 1,200-7,000 blocks) five 4.34 against 4.55;
+    // its 600x600x54 shard (y slabs of 611 blocks) 0.563 against 0.576; the
+    // 168^3 grid of a 4-cell CPML (bulk 95 %, slab launches of 105-120
+    // blocks) one launch 0.206 against 0.256 per learned-boundary step.
     const double bulk = (yl < yh && zl < zh) ? (double)(yh - yl) * (zh - zl) / ((double)(y1 - y0) * (z1 - z0)) : 0.0;
-    if (bulk >= 0.75) {
+    const int mode = launch_plan_mode();
+    if (mode != SW_PLAN_ONE && (mode == SW_PLAN_BOXES ? bulk > 0.0 : bulk >= 0.75)) {
         add_box(p, g.nx, 1, x0, x1, yl, yh, zl, zh);
         add_box(p, g.nx, 3, x0, x1, y0, yl, zl, zh);
         add_box(p, g.nx, 3, x0, x1, yh, y1, zl, zh);
         add_box(p, g.nx, 7, x0, x1, y0, y1, z0, zl);
         add_box(p, g.nx, 7, x0, x1, y0, y1, zh, z1);
-    } else {
-        add_box(p, g.nx, 7, x0, x1, y0, y1, z0, z1);
+        bool fills = true;
+        for (int t = 1; t < p.n; ++t) fills = fills && p.g[t].nblock >= compute_units();
+        if (mode == SW_PLAN_BOXES || fills) return p;
+        p = Plan();
     }
+    add_box(p, g.nx, 7, x0, x1, y0, y1, z0, z1);
     return p;
 }
 

@@ -61,6 +61,13 @@ __device__ __forceinline__ Blk xcd_block3()
 // Per-thread message of the last failure (read back by sw_last_error).
 void set_error(const char *fmt, ...);
 
+// Compute units of the current device, looked up once per device (256, the
+// MI355X's count, where the query fails, e.g. on a host without a GPU).
+int compute_units();
+
+// The launch plan mode set by sw_set_launch_plan (SW_PLAN_*).
+int launch_plan_mode();
+
 #define SW_TRY_HIP(expr)                                                              \
     do {                                                                              \
         hipError_t e_ = (expr);                                                       \

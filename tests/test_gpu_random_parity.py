@@ -1,8 +1,9 @@
 """Seeded random configurations through the three 3-D classes against the
 oracle, bit for bit: grid shape, accuracy order 1-4, anisotropic spacing,
-PML thickness and faces, source and receiver cells, class.  Sizes and PML
-thicknesses are drawn so that both launch plans occur (one launch with CPML
-on every axis, or five launches with the CPML instantiated per box)."""
+PML thickness and faces, source and receiver cells, class.  On grids this
+small the automatic launch plan is one launch with CPML on every axis; the
+five-launch plan (the CPML instantiated per box) is forced for the thin-PML
+configurations (sw_set_launch_plan)."""
 import numpy as np
 import pytest
 
@@ -69,14 +70,24 @@ def test_random_configuration_matches_oracle(seed):
     assert max(np.abs(o.fields[n]).max() for n in o.FIELDS) > 0
 
 
+@pytest.fixture
+def boxes_plan():
+    """The five-launch plan wherever the bulk box is not empty, for one test."""
+    from seiswave_taichi_amd import _lib as L
+    lib = L.load()
+    prev = lib.sw_set_launch_plan(2)  # SW_PLAN_BOXES
+    yield
+    lib.sw_set_launch_plan(prev)
+
+
 @pytest.mark.parametrize("seed", range(6))
-def test_random_five_launch_configuration_matches_oracle(seed):
-    """Seeded configurations whose PML is thin against the grid: the CPML
-    and Tensor classes run each sweep as five launches (CPML on x only in the
-    bulk, x and y in the y slabs, all axes in the z slabs), the split-field
-    class as a shell launch over its four boxes (thin x strips, y slabs, z
-    slabs) and an interior launch; orders 1-4, bit for bit against the
-    oracle, split fields included."""
+def test_random_five_launch_configuration_matches_oracle(seed, boxes_plan):
+    """Seeded configurations whose PML is thin against the grid, with the
+    five-launch plan forced: the CPML and Tensor classes run each sweep as
+    five launches (CPML on x only in the bulk, x and y in the y slabs, all
+    axes in the z slabs), the split-field class as a shell launch over its
+    four boxes (thin x strips, y slabs, z slabs) and an interior launch;
+    orders 1-4, bit for bit against the oracle, split fields included."""
     from seiswave_taichi_amd import _lib as L
     rng = np.random.default_rng(2000 + seed)
     kind = ("spml", "cpml", "tensor")[seed % 3]

@@ -361,14 +361,36 @@ def test_sweep_entry_points_refuse_the_other_class():
 
 
 def test_launch_plan_rule():
-    """One launch with CPML on every axis while the CPML-free bulk is under
-    75 % of the updated cells, five (x; x, y twice; x, y, z twice) above."""
+    """Automatic plan: five launches (x; x, y twice; x, y, z twice) when the
+    CPML-free bulk is >= 75 % of the updated cells and every slab launch has
+    at least as many blocks as the device has compute units (256 where no
+    device answers), else one launch with CPML on every axis; the two forced
+    modes; an unknown mode is refused and the mode is process-wide."""
+    lib = L.load()
     g = _fake_3d(n=60)
     for ax in (g.px, g.py, g.pz):
-        ax.lo, ax.hi = 3, 57      # thin slabs: bulk (53/56)^2 = 0.90
-    assert L.launch_plan(g) == [1, 3, 3, 7, 7]
+        ax.lo, ax.hi = 3, 57      # thin slabs: bulk (53/56)^2 = 0.90, slab launches of 13-16 blocks
+    assert L.launch_plan(g) == [7]
+    big = _fake_3d(n=400)
+    for ax in (big.px, big.py, big.pz):
+        ax.lo, ax.hi = 10, 390    # bulk (380/396)^2 = 0.92, slab launches of 1,200+ blocks
+    assert L.launch_plan(big) == [1, 3, 3, 7, 7]
+    for ax in (big.px, big.py, big.pz):
+        ax.lo, ax.hi = 3, 397     # 1-plane z slabs (about 620 blocks), 1-row y slabs (2 column blocks x 99 chunks)
+    assert L.launch_plan(big) == [7]
     for ax in (g.px, g.py, g.pz):
         ax.lo, ax.hi = 10, 50     # bulk (40/56)^2 = 0.51
+    assert L.launch_plan(g) == [7]
+    prev = lib.sw_set_launch_plan(2)  # SW_PLAN_BOXES
+    try:
+        assert prev == 0
+        assert L.launch_plan(g) == [1, 3, 3, 7, 7]
+        assert lib.sw_set_launch_plan(1) == 2  # SW_PLAN_ONE
+        assert L.launch_plan(big) == [7]
+        assert lib.sw_set_launch_plan(3) == -1
+        assert "unknown launch plan mode" in L.last_error()
+    finally:
+        lib.sw_set_launch_plan(0)
     assert L.launch_plan(g) == [7]
 
 
