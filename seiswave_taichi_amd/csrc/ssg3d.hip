@@ -8,11 +8,14 @@
 //
 // Execution: each thread owns one (i, j) column and marches along z over a
 // chunk of planes; a block takes 256 consecutive columns of a box in x-fastest
-// order, its waves starting on 128-byte lines.  On large grids a sweep is five launches over boxes of full-width
-// rows, instantiated with the CPML of only the axes whose slabs they meet (x;
-// x and y; x, y and z), so the bulk of the grid runs with fewer registers and
-// more waves in flight; on grids where the CPML shell is a large part, one
-// launch with all three axes.
+// order, its waves starting on 128-byte lines.  On large grids a sweep is
+// five launches over boxes of full-width rows, instantiated with the CPML of
+// only the axes whose slabs they meet (x; x and y; x, y and z), so the bulk
+// of the grid runs with fewer registers and more waves in flight; on grids
+// where the CPML shell is a large part, or whose slab launches could not
+// fill the GPU, one launch with all three axes.
+//  * The fields a sweep updates in place are loaded and stored
+//    non-temporally (read and written once per sweep by their owner).
 //  * The z-differentiated fields live in per-thread register windows of 2N
 //    values that slide with k, so each of their planes is read once.
 //  * Every value a plane needs (x and y stencil taps, material, the
