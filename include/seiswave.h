@@ -167,9 +167,10 @@ int64_t sw_ssg3d_split_map(const sw_ssg3d *g, int32_t sweep, int64_t *cell, int6
 /* Launch plan of the two 3-D sweeps for this descriptor (no counterpart in
  * the reference, whose loops are one Taichi task each): returns the number of
  * launches per sweep (1, or 5: the CPML-free bulk with the x CPML only, the
- * y slabs, the z slabs) and writes up to max CPML axis masks (bit 0 x, 1 y,
- * 2 z: the axes whose CPML or split-field code the launch instantiates).
- * Host only: no device access. */
+ * y slabs, the z slabs; for the split-field class 2: the shell, mask 7, and
+ * the split-free interior, mask 0) and writes up to max CPML axis masks
+ * (bit 0 x, 1 y, 2 z: the axes whose CPML or split-field code the launch
+ * instantiates).  Host only: no device access. */
 int sw_ssg3d_launch_plan(const sw_ssg3d *g, int32_t *pm, int32_t max);
 /* How the 3-D sweeps choose their launch plan, process-wide (no counterpart
  * in the reference).  SW_PLAN_AUTO (the default): five launches when the
