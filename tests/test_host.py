@@ -378,6 +378,10 @@ def test_launch_plan_rule():
     for ax in (big.px, big.py, big.pz):
         ax.lo, ax.hi = 3, 397     # 1-plane z slabs (about 620 blocks), 1-row y slabs (2 column blocks x 99 chunks)
     assert L.launch_plan(big) == [7]
+    thin = _fake_3d(n=168)        # the learned boundary's 4-cell CPML grid of the 200^3 case
+    for ax in (thin.px, thin.py, thin.pz):
+        ax.lo, ax.hi = 6, 161     # bulk (155/164)^2 = 0.89, z slabs of about 110 blocks
+    assert L.launch_plan(thin) == [7]
     for ax in (g.px, g.py, g.pz):
         ax.lo, ax.hi = 10, 50     # bulk (40/56)^2 = 0.51
     assert L.launch_plan(g) == [7]
