@@ -18,13 +18,23 @@
 //    non-temporally (read and written once per sweep by their owner).
 //  * The z-differentiated fields live in per-thread register windows of 2N
 //    values that slide with k, so each of their planes is read once.
-//  * Every value a plane needs (x and y stencil taps, material, the
+//  * Every value a plane needs (y stencil taps, material, the
 //    read-modify-write fields, CPML memory variables and coefficients) is
 //    gathered into a register "plane" before any arithmetic, and the next
 //    plane is fetched while the current one is computed: each wave keeps
-//    ~35 independent loads in flight instead of waiting on one branch at a
-//    time.  All addresses of an active column are in bounds, so the loads
-//    need no per-component predicates.
+//    ~25 independent loads in flight instead of waiting on one branch at a
+//    time.  Every lane, idle ones included, loads at an in-bounds column, so
+//    the loads need no per-component predicates.
+//  * x taps, and the material at p+1, come from the neighbouring lanes (DPP
+//    wave shifts), the two wave-edge values from scalar loads; the y taps'
+//    centres come from the z windows.  Both sweeps then fit 4 waves per SIMD
+//    at 200^3 (120 / 124 VGPRs against 137 / 147 with 16-byte x-tap loads):
+//    measured on MI355X in one process, bit-identical, 200^3 0.2855 ->
+//    0.2734 ms per step, 600x600x400 4.342 -> 4.125 ms.  The same shifts with
+//    the edge values from vector loads ran 0.298-0.343 ms: a wait for those
+//    loads is a wait for the whole prefetched plane (vmcnt counts in order,
+//    and the plane's lane-predicated slab loads leave the compiler no count
+//    but 0), and each adds a vector-memory instruction.
 //  * Blocks are renumbered so that the blocks of one neighbourhood run on the
 //    same XCD (MI355X dispatches consecutive workgroups round-robin over 8
 //    XCDs, each with its own L2) and halo lines are shared within one L2.
@@ -35,10 +45,6 @@
 
 #include "ssg3d_common.h"
 
-// SW_PREFETCH=1 double-buffers the register plane (fetch k+1 while computing k).
-#ifndef SW_PREFETCH
-#define SW_PREFETCH 1
-#endif
 namespace sw {
 
 namespace {
@@ -59,8 +65,11 @@ struct Geo {
     int nblock;
 };
 
+// i, j: the lane's cell column, -1 on idle lanes; qi, qj: the column of the
+// lane's flat index, idle lanes included (they keep loading for their
+// neighbours' x taps).
 struct Cell {
-    int i, j, k0, k1;
+    int i, j, qi, qj, k0, k1;
 };
 
 __device__ __forceinline__ Cell box_cell(const Geo &geo)
@@ -77,6 +86,8 @@ __device__ __forceinline__ Cell box_cell(const Geo &geo)
     const int q = B.f0 + cb * kThreads + (int)threadIdx.x;
     const int j = q / B.nx, i = q - j * B.nx;
     Cell c;
+    c.qi = i;
+    c.qj = j;
     if (q >= B.fn || j < B.j0 || j >= B.j0 + B.h || i < B.i0 || i >= B.i0 + B.w) {
         c.i = -1; c.j = -1;
     } else {
@@ -98,25 +109,30 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
     const Cell cl = box_cell(geo);
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i < N || i >= nx - N || j < N || j >= ny - N) return;
+    // Idle lanes run the loop without storing: they load at their own
+    // column (its row moved into [N, ny-N) if outside, where no updated
+    // cell uses it), so every lane can serve its neighbours' x taps.
+    const bool live = i >= 0;
+    const int li = cl.qi, lj = min(max(cl.qj, N), ny - N - 1);
     const int SY = 4 * nx;
     const uint32_t SZ = 4u * (uint32_t)(nx * ny);
     const float dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
     float c[N];
 #pragma unroll
     for (int t = 0; t < N; ++t) c[t] = g.c[t];
-    const bool vx_col = i < nx - N - 1 && j >= N + 1;
-    const bool vy_col = j < ny - N - 1 && i >= N + 1;
-    const bool vz_col = i >= N + 1 && j >= N + 1;
+    const bool vx_col = live && i < nx - N - 1 && j >= N + 1;
+    const bool vy_col = live && j < ny - N - 1 && i >= N + 1;
+    const bool vz_col = live && i >= N + 1 && j >= N + 1;
     // CPML axes of this instance (none for the split-field class, whose
     // kernels here run only its split-free interior)
     constexpr int CPM = CLS == SW_CLASS_SPML ? 0 : PM;
-    const Slab sl = slab_setup<CPM>(g, i, j);
+    Slab sl = slab_setup<CPM>(g, li, lj);
+    if (!live) sl.xs = sl.ys = -1;
     const int kb = LARGE ? cl.k0 - N : 0;  // base plane of this block's offsets
     const Res R = make_res(g, kb);
-    const Coef cx = coef_of(g.px, i, sl.xs >= 0);
-    const Coef cy = coef_of(g.py, j, sl.ys >= 0);
-    const uint32_t pc = 4u * (uint32_t)(j * nx + i);
+    const Coef cx = coef_of(g.px, li, sl.xs >= 0);
+    const Coef cy = coef_of(g.py, lj, sl.ys >= 0);
+    const uint32_t pc = 4u * (uint32_t)(lj * nx + li);
     // windows: wxz[t] = sxz[k-N+t], wyz[t] = syz[k-N+t], wzz[t] = szz[k-N+1+t]
     float wxz[2 * N], wyz[2 * N], wzz[2 * N];
     const int k0 = cl.k0;
@@ -126,26 +142,31 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
         wyz[t] = ld(R.syz, pc + (uint32_t)(k0 - N + t - kb) * SZ);
         wzz[t] = ld(R.szz, pc + (uint32_t)(k0 - N + 1 + t - kb) * SZ);
     }
+    // the register plane is double-buffered: plane k+1 is fetched while k is computed
     VPlane<N> P;
-#if SW_PREFETCH
     load_vplane<N, CPM>(P, g, R, pc, k0, kb, SY, SZ, sl);
-#endif
+    const Edge e0 = wave_edge(pc);  // lane 0's and lane 63's plane offsets
     for (int k = k0; k < cl.k1; ++k) {
-#if SW_PREFETCH
-        VPlane<N> Q;
-        if (k + 1 < cl.k1) load_vplane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
-#else
-        load_vplane<N, CPM>(P, g, R, pc, k, kb, SY, SZ, sl);
-#endif
         const uint32_t p = pc + (uint32_t)(k - kb) * SZ;
-        const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
-        const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+        const Edge e{e0.lo + (uint32_t)(k - kb) * SZ, e0.hi + (uint32_t)(k - kb) * SZ};
         wxz[2 * N - 1] = P.wxz;
         wyz[2 * N - 1] = P.wyz;
         wzz[2 * N - 1] = P.wzz;
+        VPlane<N> Q;
+        if (k + 1 < cl.k1) load_vplane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
+        // x taps and the p+1 densities from the neighbouring lanes (all lanes active)
+        P.syz_y[N] = wyz[N];  // syz at p, from the window
+        float sxx_x[2 * N], sxy_x[2 * N], sxz_x[2 * N];
+        xtaps_fm_lanes<N>(sxx_x, P.sxx, R.sxx, e);
+        xtaps_bm_lanes<N>(sxy_x, P.sxy_y[N], R.sxy, e);
+        xtaps_bm_lanes<N>(sxz_x, wxz[N], R.sxz, e);
+        const float rho_x = next_x(P.rho[0], R.rho, e);                           // p+1
+        const float rho_xy = next_x(P.rho[1], R.rho, Edge{e.lo + SY, e.hi + SY});  // p+1+SY
+        const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+        const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
         // vx at (i+1/2, j, k): src/Elastic3D_SSG_CPML.py:164-178
         if (vx_col && k >= N + 1) {
-            float dsxx = diff_fm<N>(P.sxx, c) / dx;
+            float dsxx = diff_fm<N>(sxx_x, c) / dx;
             float dsxy = diff_bm<N>(P.sxy_y, c) / dy;
             float dsxz = diff_bm<N>(wxz, c) / dz;
             if (sl.xs >= 0) {
@@ -168,7 +189,7 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
         // vy at (i, j+1/2, k): :180-195
         if (vy_col && k >= N + 1) {
             const float rho_half_y = 0.5f * (P.rho[0] + P.rho[1]);
-            float dsxy = diff_bm<N>(P.sxy_x, c) / dx;
+            float dsxy = diff_bm<N>(sxy_x, c) / dx;
             float dsyy = diff_fm<N>(P.syy, c) / dy;
             float dsyz = diff_bm<N>(wyz, c) / dz;
             if (sl.xs >= 0) {
@@ -190,8 +211,8 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
         }
         // vz at (i, j, k+1/2): :197-212 (Tensor :228-243)
         if (vz_col && k < nz - N - 1) {
-            const float rho_avg = 0.25f * (P.rho[0] + P.rho[1] + P.rho[2] + P.rho[3]);
-            float dsxz = diff_bm<N>(P.sxz_x, c) / dx;
+            const float rho_avg = 0.25f * (P.rho[0] + P.rho[1] + rho_xy + rho_x);
+            float dsxz = diff_bm<N>(sxz_x, c) / dx;
             float dsyz = diff_bm<N>(P.syz_y, c) / dy;
             float dszz = diff_fm<N>(wzz, c) / dz;
             if (sl.xs >= 0) {
@@ -216,9 +237,7 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
         win_shift<N>(wxz);
         win_shift<N>(wyz);
         win_shift<N>(wzz);
-#if SW_PREFETCH
         P = Q;
-#endif
     }
     // folded force source: after the loop (no other thread of this sweep
     // reads the cell's velocity), which keeps it out of the loop's registers
@@ -229,14 +248,11 @@ __global__ void __launch_bounds__(kThreads) k_velocity(sw_ssg3d g, Geo geo, Fold
 // -------------------------------------------------------------- stress sweep
 template <int N>
 struct SPlane {
-    float vx_x[2 * N];  // bm along x (Dx3bm, normal)
-    float vx_y[2 * N];  // fm along y (Dy3fm, sxy)
-    float vy_x[2 * N];  // fm along x (Dx3fm, sxy)
-    float vy_y[2 * N];  // bm along y (Dy3bm, normal)
-    float vz_x[2 * N];  // fm along x (Dx3fm, sxz)
+    float vx_y[2 * N];  // fm along y (Dy3fm, sxy); centres from the windows,
+    float vy_y[2 * N];  // bm along y (Dy3bm, normal)  x taps from the lanes
     float vz_y[2 * N];  // fm along y (Dy3fm, syz)
-    float lam[2];       // p, p+1
-    float mu[4];        // p, p+1, p+SY, p+SZ
+    float lam;          // p (p+1 from the next lane)
+    float mu[3];        // p, p+SY, p+SZ (p+1 from the next lane)
     float s[6];         // sxx syy szz sxy sxz syz at p
     float wvx, wvy, wvz; // next window entries: vx, vy at k+N; vz at k+N-1
     float mx[3], my[3], mz[3];
@@ -250,18 +266,14 @@ __device__ __forceinline__ void load_splane(SPlane<N> &P, const sw_ssg3d &g, con
 {
     const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)(k - kb) * SZ;
     const int c0 = N * 4 + N * SY;
-    xtaps_bm<N>(P.vx_x, R.vx, vb, c0);
-    taps_fm<N>(P.vx_y, R.vx, vb, c0, SY);
-    xtaps_fm<N>(P.vy_x, R.vy, vb, c0);
-    taps_bm<N>(P.vy_y, R.vy, vb, c0, SY);
-    xtaps_fm<N>(P.vz_x, R.vz, vb, c0);
-    taps_fm<N>(P.vz_y, R.vz, vb, c0, SY);
-    P.lam[0] = ldu(R.lam, vb, c0);
-    P.lam[1] = ldu(R.lam, vb, c0 + 4);
+    // y taps without their centres: the z windows hold vx, vy, vz at p
+    taps_fm_nc<N>(P.vx_y, R.vx, vb, c0, SY);
+    taps_bm_nc<N>(P.vy_y, R.vy, vb, c0, SY);
+    taps_fm_nc<N>(P.vz_y, R.vz, vb, c0, SY);
+    P.lam = ldu(R.lam, vb, c0);
     P.mu[0] = ldu(R.mu, vb, c0);
-    P.mu[1] = ldu(R.mu, vb, c0 + 4);
-    P.mu[2] = ldu(R.mu, vb, c0 + SY);
-    P.mu[3] = ldu(R.mu, vb, c0 + (int)SZ);
+    P.mu[1] = ldu(R.mu, vb, c0 + SY);
+    P.mu[2] = ldu(R.mu, vb, c0 + (int)SZ);
     P.s[0] = ldu_nt(R.sxx, vb, c0);
     P.s[1] = ldu_nt(R.syy, vb, c0);
     P.s[2] = ldu_nt(R.szz, vb, c0);
@@ -294,24 +306,27 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
     const Cell cl = box_cell(geo);
     const int i = cl.i, j = cl.j;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i < N || i >= nx - N || j < N || j >= ny - N) return;
+    // idle lanes load at their own (row-clamped) column and store nothing
+    const bool live = i >= 0;
+    const int li = cl.qi, lj = min(max(cl.qj, N), ny - N - 1);
     const int SY = 4 * nx;
     const uint32_t SZ = 4u * (uint32_t)(nx * ny);
     const float dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
     float c[N];
 #pragma unroll
     for (int t = 0; t < N; ++t) c[t] = g.c[t];
-    const bool nrm_col = i >= N + 1 && j >= N + 1;
-    const bool sxy_col = i >= N + 1 && j >= N + 1;
-    const bool sxz_col = i < nx - N - 1;
-    const bool syz_col = j < ny - N - 1;
+    const bool nrm_col = live && i >= N + 1 && j >= N + 1;
+    const bool sxy_col = live && i >= N + 1 && j >= N + 1;
+    const bool sxz_col = live && i < nx - N - 1;
+    const bool syz_col = live && j < ny - N - 1;
     constexpr int CPM = CLS == SW_CLASS_SPML ? 0 : PM;
-    const Slab sl = slab_setup<CPM>(g, i, j);
+    Slab sl = slab_setup<CPM>(g, li, lj);
+    if (!live) sl.xs = sl.ys = -1;
     const int kb = LARGE ? cl.k0 - N : 0;  // base plane of this block's offsets
     const Res R = make_res(g, kb);
-    const Coef cx = coef_of(g.px, i, sl.xs >= 0);
-    const Coef cy = coef_of(g.py, j, sl.ys >= 0);
-    const uint32_t pc = 4u * (uint32_t)(j * nx + i);
+    const Coef cx = coef_of(g.px, li, sl.xs >= 0);
+    const Coef cy = coef_of(g.py, lj, sl.ys >= 0);
+    const uint32_t pc = 4u * (uint32_t)(lj * nx + li);
     // windows: wvz[t] = vz[k-N+t] (backward), wvx[t] = vx[k-N+1+t], wvy[t] = vy[k-N+1+t] (forward)
     float wvz[2 * N], wvx[2 * N], wvy[2 * N];
     const int k0 = cl.k0;
@@ -322,30 +337,37 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
         wvy[t] = ld(R.vy, pc + (uint32_t)(k0 - N + 1 + t - kb) * SZ);
     }
     SPlane<N> P;
-#if SW_PREFETCH
     load_splane<N, CPM>(P, g, R, pc, k0, kb, SY, SZ, sl);
-#endif
+    const Edge e0 = wave_edge(pc);
     for (int k = k0; k < cl.k1; ++k) {
-#if SW_PREFETCH
-        SPlane<N> Q;
-        if (k + 1 < cl.k1) load_splane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
-#else
-        load_splane<N, CPM>(P, g, R, pc, k, kb, SY, SZ, sl);
-#endif
         const uint32_t p = pc + (uint32_t)(k - kb) * SZ;
-        const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
-        const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
+        const Edge e{e0.lo + (uint32_t)(k - kb) * SZ, e0.hi + (uint32_t)(k - kb) * SZ};
         wvz[2 * N - 1] = P.wvz;
         wvx[2 * N - 1] = P.wvx;
         wvy[2 * N - 1] = P.wvy;
+        SPlane<N> Q;
+        if (k + 1 < cl.k1) load_splane<N, CPM>(Q, g, R, pc, k + 1, kb, SY, SZ, sl);
+        // centres of the y taps from the windows (vx, vy, vz at p)
+        P.vx_y[N - 1] = wvx[N - 1];
+        P.vy_y[N] = wvy[N - 1];
+        P.vz_y[N - 1] = wvz[N];
+        // x taps and the p+1 moduli from the neighbouring lanes (all lanes active)
+        float vx_x[2 * N], vy_x[2 * N], vz_x[2 * N];
+        xtaps_bm_lanes<N>(vx_x, wvx[N - 1], R.vx, e);
+        xtaps_fm_lanes<N>(vy_x, wvy[N - 1], R.vy, e);
+        xtaps_fm_lanes<N>(vz_x, wvz[N], R.vz, e);
+        const float lam_x = next_x(P.lam, R.lam, e);   // p+1
+        const float mu_x = next_x(P.mu[0], R.mu, e);  // p+1
+        const uint32_t ox = sl.rx + (uint32_t)k * sl.sx, oy = sl.ry + (uint32_t)k * sl.sy;
+        const uint32_t oz = (uint32_t)P.zs * sl.sz + pc;
         // normal stresses at (i, j, k): src/Elastic3D_SSG_CPML.py:247-266
         if (nrm_col && k >= N + 1) {
             // CPML/Tensor average lambda, mu over (i, i+1); SPML takes the
             // point values (src/Elastic3D_SSG_SPML.py:300-302)
-            const float lam_h = CLS == SW_CLASS_SPML ? P.lam[0] : 0.5f * (P.lam[0] + P.lam[1]);
-            const float mu_h = CLS == SW_CLASS_SPML ? P.mu[0] : 0.5f * (P.mu[0] + P.mu[1]);
+            const float lam_h = CLS == SW_CLASS_SPML ? P.lam : 0.5f * (P.lam + lam_x);
+            const float mu_h = CLS == SW_CLASS_SPML ? P.mu[0] : 0.5f * (P.mu[0] + mu_x);
             const float lpm = lam_h + 2.f * mu_h;
-            float dvxx = diff_bm<N>(P.vx_x, c) / dx;
+            float dvxx = diff_bm<N>(vx_x, c) / dx;
             float dvyy = diff_bm<N>(P.vy_y, c) / dy;
             float dvzz = diff_bm<N>(wvz, c) / dz;
             if (sl.xs >= 0) {
@@ -366,8 +388,8 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
         }
         // sxy at (i+1/2, j+1/2, k): :267-279
         if (sxy_col) {
-            const float mu_a = 0.25f * (P.mu[2] + P.mu[0] + P.mu[0] + P.mu[1]);
-            float dvyx = diff_fm<N>(P.vy_x, c) / dx;
+            const float mu_a = 0.25f * (P.mu[1] + P.mu[0] + P.mu[0] + mu_x);
+            float dvyx = diff_fm<N>(vy_x, c) / dx;
             float dvxy = diff_fm<N>(P.vx_y, c) / dy;
             if (sl.xs >= 0) {
                 dvyx = cpml_step(dvyx, P.mx[1], cx.ah, cx.bh, cx.kh);
@@ -381,8 +403,8 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
         }
         // sxz at (i+1/2, j, k+1/2): :281-293
         if (sxz_col && k < nz - N - 1) {
-            const float mu_a = 0.25f * (P.mu[0] + P.mu[1] + P.mu[0] + P.mu[2]);
-            float dvzx = diff_fm<N>(P.vz_x, c) / dx;
+            const float mu_a = 0.25f * (P.mu[0] + mu_x + P.mu[0] + P.mu[1]);
+            float dvzx = diff_fm<N>(vz_x, c) / dx;
             float dvxz = diff_fm<N>(wvx, c) / dz;
             if (sl.xs >= 0) {
                 dvzx = cpml_step(dvzx, P.mx[2], cx.ah, cx.bh, cx.kh);
@@ -396,7 +418,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
         }
         // syz at (i, j+1/2, k+1/2): :295-307
         if (syz_col && k < nz - N - 1) {
-            const float mu_a = 0.25f * (P.mu[2] + P.mu[0] + P.mu[3] + P.mu[0]);
+            const float mu_a = 0.25f * (P.mu[1] + P.mu[0] + P.mu[2] + P.mu[0]);
             float dvyz = diff_fm<N>(wvy, c) / dz;
             float dvzy = diff_fm<N>(P.vz_y, c) / dy;
             if (sl.ys >= 0) {
@@ -412,9 +434,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(sw_ssg3d g, Geo geo, Fold3D
         win_shift<N>(wvz);
         win_shift<N>(wvx);
         win_shift<N>(wvy);
-#if SW_PREFETCH
         P = Q;
-#endif
     }
     // folded next-step stress source, after the loop (no other thread of
     // this sweep reads stresses)

@@ -1,6 +1,6 @@
-// Device helpers of the 3-D sweep kernels (ssg3d.hip): addressing, stencil
-// taps, staggered differences, CPML slabs and the velocity-sweep register
-// plane.  They sit in an anonymous namespace, so a translation unit that
+// Device helpers of the 3-D sweep kernels (ssg3d.hip, ssg3d_split.hip):
+// addressing, stencil taps (loaded, or shifted in from neighbouring lanes),
+// staggered differences, CPML slabs and the velocity-sweep register plane.  They sit in an anonymous namespace, so a translation unit that
 // includes them keeps its own copy.
 #pragma once
 
@@ -113,9 +113,29 @@ __device__ __forceinline__ void taps_fm(float *w, const float *__restrict__ f, u
     for (int t = 0; t < 2 * N; ++t) w[t] = ldu(f, vb, c0 + (t - N + 1) * st);
 }
 
-// x taps are 2N consecutive floats: one 16-byte (or 8-byte) load per 4 (or 2)
-// taps instead of one load each.  The addresses are only 4-byte aligned,
-// which global vector loads accept.
+// The same without the centre (offset 0: w[N] of a bm array, w[N-1] of an
+// fm array), for fields whose z window already holds it; the kernel fills
+// it in before use.
+template <int N>
+__device__ __forceinline__ void taps_bm_nc(float *w, const float *__restrict__ f, uint32_t vb, int c0, int st)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t)
+        if (t != N) w[t] = ldu(f, vb, c0 + (t - N) * st);
+}
+
+template <int N>
+__device__ __forceinline__ void taps_fm_nc(float *w, const float *__restrict__ f, uint32_t vb, int c0, int st)
+{
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t)
+        if (t != N - 1) w[t] = ldu(f, vb, c0 + (t - N + 1) * st);
+}
+
+// x taps of the split-field shell kernels (ssg3d_split.hip), whose lanes do
+// not always hold consecutive columns: 2N consecutive floats, one 16-byte (or
+// 8-byte) load per 4 (or 2) taps instead of one load each.  The addresses
+// are only 4-byte aligned, which global vector loads accept.
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 struct __attribute__((packed, aligned(4))) F4U { f4v v; };
@@ -150,6 +170,77 @@ template <int N>
 __device__ __forceinline__ void xtaps_fm(float *w, const float *__restrict__ f, uint32_t vb, int c0)
 {
     ld_run<2 * N>(w, f, vb, c0 + (1 - N) * 4);
+}
+
+// x taps from the neighbouring lanes (the plain sweeps).  A wave's 64 lanes
+// hold 64 consecutive columns of a row-major plane, so the x neighbour at
+// distance d of lane l sits in lane l - d or l + d, except past lane 0 or 63.
+// Each lane keeps its own value; wave_shr:1 (DPP control 0x138) gives lane l
+// the value of lane l - 1 and wave_shl:1 (0x130) that of lane l + 1, and the
+// lane without a source (lane 0, resp. 63) takes the instruction's `old`
+// operand.  The values lanes 0 and 63 lack are wave-uniform, so `old` is a
+// scalar: the field at lane 0's offset minus 4d or lane 63's plus 4d, read
+// through the scalar cache (s_load, no vector-memory instruction; the sweep
+// never writes the arrays it reads this way).  Every lane of the wave must
+// be active at the shifts, and every lane whose value a cell uses holds its
+// own column's value (the kernels keep idle lanes running).  This replaces
+// each field's 16-byte x-tap load and the p+1 loads of the material, and the
+// register plane no longer carries the x taps.
+__device__ __forceinline__ float from_prev_lane(float v, float old)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float from_next_lane(float v, float old)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// Wave-uniform load at byte offset off through the scalar cache.
+typedef const __attribute__((address_space(4))) float sfloat;
+__device__ __forceinline__ float lds_u(const float *__restrict__ base, uint32_t off)
+{
+    return *(sfloat *)(reinterpret_cast<const char *>(base) + off);  // address-space cast
+}
+
+// Edge offsets of a wave: the byte offsets of lane 0's and lane 63's cells
+// (call with every lane active).  Both columns have N rows of the plane on
+// either side, so the N floats before and after them are inside the array.
+struct Edge {
+    uint32_t lo, hi;
+};
+
+__device__ __forceinline__ Edge wave_edge(uint32_t v)
+{
+    return Edge{(uint32_t)__builtin_amdgcn_readfirstlane(v), (uint32_t)__builtin_amdgcn_readlane(v, 63)};
+}
+
+// bm taps f[q + t - N], fm taps f[q + t - N + 1], t = 0..2N-1, of field f at
+// the plane whose edge offsets are e; own = f[q].
+template <int N>
+__device__ __forceinline__ void xtaps_bm_lanes(float *w, float own, const float *__restrict__ f, Edge e)
+{
+    w[N] = own;
+#pragma unroll
+    for (int t = N - 1; t >= 0; --t) w[t] = from_prev_lane(w[t + 1], lds_u(f, e.lo - 4u * (N - t)));
+#pragma unroll
+    for (int t = N + 1; t < 2 * N; ++t) w[t] = from_next_lane(w[t - 1], lds_u(f, e.hi + 4u * (t - N)));
+}
+
+template <int N>
+__device__ __forceinline__ void xtaps_fm_lanes(float *w, float own, const float *__restrict__ f, Edge e)
+{
+    w[N - 1] = own;
+#pragma unroll
+    for (int t = N - 2; t >= 0; --t) w[t] = from_prev_lane(w[t + 1], lds_u(f, e.lo - 4u * (N - 1 - t)));
+#pragma unroll
+    for (int t = N; t < 2 * N; ++t) w[t] = from_next_lane(w[t - 1], lds_u(f, e.hi + 4u * (t - N + 1)));
+}
+
+// f[q + 1] of a value the lane holds as own = f[q].
+__device__ __forceinline__ float next_x(float own, const float *__restrict__ f, Edge e)
+{
+    return from_next_lane(own, lds_u(f, e.hi + 4u));
 }
 
 // Staggered differences from tap arrays, accumulated exactly as
@@ -239,13 +330,11 @@ __device__ __forceinline__ float cpml_step(float D, float &m, float a, float b, 
 // ------------------------------------------------------------ velocity sweep
 template <int N>
 struct VPlane {
-    float sxx[2 * N];   // fm taps along x (Dx3fm, vx)
-    float sxy_x[2 * N]; // bm taps along x (Dx3bm, vy)
+    float sxx;          // sxx at p (its x taps come from the neighbouring lanes)
     float sxy_y[2 * N]; // bm taps along y (Dy3bm, vx)
     float syy[2 * N];   // fm taps along y (Dy3fm, vy)
-    float sxz_x[2 * N]; // bm taps along x (Dx3bm, vz)
-    float syz_y[2 * N]; // bm taps along y (Dy3bm, vz)
-    float rho[4];       // p, p+SY, p+1+SY, p+1
+    float syz_y[2 * N]; // bm taps along y (Dy3bm, vz); centre from the window
+    float rho[2];       // p, p+SY (p+1, p+1+SY from the next lane)
     float v[3];
     float wxz, wyz, wzz; // next window entries: sxz, syz at k+N-1, szz at k+N
     float mx[3], my[3], mz[3];
@@ -262,16 +351,12 @@ __device__ __forceinline__ void load_vplane(VPlane<N> &P, const sw_ssg3d &g, con
     // vb: byte offset of (i-N, j-N, k); every load below is vb + a uniform delta
     const uint32_t vb = pc - (uint32_t)(N * 4 + N * SY) + (uint32_t)(k - kb) * SZ;
     const int c0 = N * 4 + N * SY;
-    xtaps_fm<N>(P.sxx, R.sxx, vb, c0);
-    xtaps_bm<N>(P.sxy_x, R.sxy, vb, c0);
+    P.sxx = ldu(R.sxx, vb, c0);
     taps_bm<N>(P.sxy_y, R.sxy, vb, c0, SY);
     taps_fm<N>(P.syy, R.syy, vb, c0, SY);
-    xtaps_bm<N>(P.sxz_x, R.sxz, vb, c0);
-    taps_bm<N>(P.syz_y, R.syz, vb, c0, SY);
+    taps_bm_nc<N>(P.syz_y, R.syz, vb, c0, SY);  // centre: the syz window
     P.rho[0] = ldu(R.rho, vb, c0);
     P.rho[1] = ldu(R.rho, vb, c0 + SY);
-    P.rho[2] = ldu(R.rho, vb, c0 + 4 + SY);
-    P.rho[3] = ldu(R.rho, vb, c0 + 4);
     P.v[0] = ldu_nt(R.vx, vb, c0);
     P.v[1] = ldu_nt(R.vy, vb, c0);
     P.v[2] = ldu_nt(R.vz, vb, c0);
