@@ -500,6 +500,9 @@ Plan make_plan(const sw_ssg3d &g, int N)
     // its 600x600x54 shard (y slabs of 611 blocks) 0.563 against 0.576; the
     // 168^3 grid of a 4-cell CPML (bulk 95 %, slab launches of 105-120
     // blocks) one launch 0.206 against 0.256 per learned-boundary step.
+    // With the x taps from neighbouring lanes (4 waves per SIMD in the
+    // PM=7 kernels, 5 in PM=1): 200^3 one 0.274 against 0.333 in five,
+    // 600x600x400 five 4.132 against 4.233, the shard 0.534 either way.
     const double bulk = (yl < yh && zl < zh) ? (double)(yh - yl) * (zh - zl) / ((double)(y1 - y0) * (z1 - z0)) : 0.0;
     const int mode = launch_plan_mode();
     if (mode != SW_PLAN_ONE && (mode == SW_PLAN_BOXES ? bulk > 0.0 : bulk >= 0.75)) {
