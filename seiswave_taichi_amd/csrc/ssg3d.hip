@@ -468,7 +468,8 @@ struct Plan {
 // with chunks of 2 / 4 / 6 / 8 planes, 200^3 0.309 / 0.305 / 0.313 / 0.317,
 // 600x600x400 4.77 (4) / 4.92 (8); with line-aligned columns and chunks of
 // 4 / 6 / 8 planes, 200^3 0.289 / 0.289 / 0.298, 600x600x400 4.42 / 4.47 /
-// 4.49, 200^3 split-field class 0.425 / 0.436 / 0.450.
+// 4.49, 200^3 split-field class 0.425 / 0.436 / 0.450; with the x taps from
+// neighbouring lanes 3-plane chunks (kGroup has the figures).
 void add_box(Plan &p, int nx, int pm, int i0, int i1, int j0, int j1, int k0, int k1)
 {
     if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;

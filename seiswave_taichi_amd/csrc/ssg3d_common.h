@@ -19,7 +19,7 @@ namespace {
 constexpr int kThreads = 256;
 
 // Blocks run in groups of kGroup column blocks (8,192 columns) that march
-// through all z chunks before the next group starts.  With 4-plane chunks a
+// through all z chunks before the next group starts.  With short chunks a
 // chunk's window planes are the ones the same group read one chunk earlier,
 // still in the XCD's L2, while a group is wide enough that its y halo rows
 // are a small part of its reads.  Measured on MI355X, ms per step with groups
@@ -31,7 +31,14 @@ constexpr int kThreads = 256;
 // 0.425 / 0.422; groups of 32 / 64 / 128 / 256 on 600x600x400 4.429 /
 // 4.385 / 4.374 / 4.414 and on its 600x600x54 shard 0.577 / 0.581 / 0.577 /
 // 0.589 (one group size for every grid: the gains elsewhere stay near 1 %).
-constexpr int kGroup = 32;
+// With the x taps from neighbouring lanes (4 waves per SIMD), chunks of 3 / 4
+// / 5 planes in groups of 32 / 48 / 64 column blocks, ms per step (one
+// process, bit-identical): 200^3 4x32 0.2744, 3x32 0.2723, 4x64 0.2749,
+// 3x64 0.2717, 3x48 0.2732, 4x48 0.2778, 5x32 0.2783 (another box); 600x600x400
+// 4.141 / 4.136 / 4.069 / 4.052 / 4.090 / 4.097; split-field class 0.4161 /
+// 0.4043 / 0.4125 / 0.4024 / 0.4043 / 0.4151; 600x600x54 shard 0.5341 /
+// 0.5303 / 0.5335 / 0.5312 / 0.5312 / 0.5340: 3-plane chunks in groups of 64.
+constexpr int kGroup = 64;
 
 // Fields are addressed by 32-bit byte offsets from their base pointers: one
 // offset register per column serves every field, and the wave-uniform tap

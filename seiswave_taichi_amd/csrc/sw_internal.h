@@ -14,7 +14,7 @@ namespace sw {
 
 // Planes per z chunk of the 3-D sweeps: a block marches its columns over one
 // chunk, and its 32-bit offsets span the chunk plus the stencil halo.
-constexpr int kZChunk = 4;
+constexpr int kZChunk = 3;
 
 constexpr int kXCD = 8;  // XCDs per MI355X; any value keeps the renumbering a bijection
 

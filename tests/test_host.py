@@ -320,7 +320,7 @@ def test_grid_size_limits_are_checked():
 
     rc, msg = run(1300, 1300, 1300)  # 2.2e9 cells
     assert rc == -1 and "int32 cell index" in msg
-    rc, msg = run(11000, 11000, 8)  # 121e6-cell planes: a chunk of 4 planes + halo exceeds 4 GiB
+    rc, msg = run(12000, 12000, 8)  # 144e6-cell planes: a 3-plane chunk + halo exceeds 4 GiB
     assert rc == -1 and "z chunk" in msg
     rc, msg = run(200, 3000, 3000, lo=400)  # 800-cell x slabs: 24*800*3000*3000 B
     assert rc == -1 and "memory slabs" in msg
@@ -375,9 +375,10 @@ def test_launch_plan_rule():
     for ax in (big.px, big.py, big.pz):
         ax.lo, ax.hi = 10, 390    # bulk (380/396)^2 = 0.92, slab launches of 1,200+ blocks
     assert L.launch_plan(big) == [1, 3, 3, 7, 7]
-    for ax in (big.px, big.py, big.pz):
-        ax.lo, ax.hi = 3, 397     # 1-plane z slabs (about 620 blocks), 1-row y slabs (2 column blocks x 99 chunks)
-    assert L.launch_plan(big) == [7]
+    mid = _fake_3d(n=360)
+    for ax in (mid.px, mid.py, mid.pz):
+        ax.lo, ax.hi = 3, 357     # 1-plane z slabs (about 500 blocks), 1-row y slabs (2 column blocks x 118 chunks)
+    assert L.launch_plan(mid) == [7]
     thin = _fake_3d(n=168)        # the learned boundary's 4-cell CPML grid of the 200^3 case
     for ax in (thin.px, thin.py, thin.pz):
         ax.lo, ax.hi = 6, 161     # bulk (155/164)^2 = 0.89, z slabs of about 110 blocks
